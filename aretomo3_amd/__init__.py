@@ -1,0 +1,17 @@
+"""aretomo3_amd -- MI355X-native (gfx950) rebuild of AreTomo3's tomographic
+reconstruction hot path: projection weighting and ramp/Hamming filter,
+weighted back-projection, and the SART forward/difference/back-projection
+loop, with the stack-correction pre-stage that feeds them.
+
+All computation runs in the HIP library ``libatr_recon.so`` (C ABI in
+``include/aretomo_recon.h``); this package is the host-side mirror of the
+reference operators (``recon``) and the synthetic-data generator
+(``phantom``).
+"""
+from . import _lib
+from ._lib import AtrError
+from .recon import (DeviceArray, DoSartRecon, DoWbpRecon, Plan, TomoSart, TomoWbp,
+                    device_count, synchronize)
+
+__all__ = ["AtrError", "DeviceArray", "DoSartRecon", "DoWbpRecon", "Plan", "TomoSart",
+           "TomoWbp", "device_count", "synchronize", "_lib"]

@@ -1,0 +1,130 @@
+"""ctypes binding of libatr_recon.so (declared in include/aretomo_recon.h).
+
+The library is loaded from this package directory only.  If it is missing the
+import fails loudly: there is no CPU fallback anywhere in this package.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libatr_recon.so")
+
+ATR_OK = 0
+STATUS_NAMES = {0: "ATR_OK", 1: "ATR_ERR_INVALID_ARG", 2: "ATR_ERR_OUT_OF_MEMORY",
+                3: "ATR_ERR_HIP", 4: "ATR_ERR_UNSUPPORTED_SIZE"}
+
+METHOD_WBP, METHOD_SART = 0, 1
+STORAGE_F32, STORAGE_F16, STORAGE_BF16 = 0, 1, 2
+SUBSETS_CONTIGUOUS, SUBSETS_INTERLEAVED = 0, 1
+LAYOUT_XZY, LAYOUT_XYZ, LAYOUT_XYZ_NOFLIP, LAYOUT_SLICE = 0, 1, 2, 3
+
+# kernel ids of atr_plan_kernel_stats
+K_BP, K_FP, K_WEIGHT, K_WBP_BP, K_PRESTAGE, K_OUTPUT, K_WBP_COL, K_WBP_ROW = range(8)
+
+# every symbol include/aretomo_recon.h declares
+EXPORTS = [
+    "atr_series_desc_init", "atr_plan_create", "atr_plan_get_volume_size",
+    "atr_plan_device_bytes", "atr_estimate_device_bytes", "atr_execute", "atr_execute_host",
+    "atr_plan_set_profiling", "atr_plan_kernel_stats", "atr_plan_reset_stats",
+    "atr_plan_destroy", "atr_last_error_string", "atr_version", "atr_phantom_tilt_series",
+    "atr_device_count", "atr_set_device", "atr_malloc", "atr_free", "atr_memcpy_h2d",
+    "atr_memcpy_d2h", "atr_memset", "atr_synchronize", "atr_stream_create",
+    "atr_stream_destroy", "atr_stream_synchronize", "atr_mem_info", "atr_event_create",
+    "atr_event_destroy", "atr_event_record", "atr_event_elapsed_ms",
+]
+
+
+class SeriesDesc(ctypes.Structure):
+    """Mirror of atr_series_desc."""
+    _fields_ = [
+        ("proj_x", ctypes.c_int),
+        ("proj_y", ctypes.c_int),
+        ("n_tilts", ctypes.c_int),
+        ("tilt_stride", ctypes.c_size_t),
+        ("row_stride", ctypes.c_size_t),
+        ("tilt_deg", ctypes.c_void_p),
+        ("tilt_axis_deg", ctypes.c_void_p),
+        ("shift_xy", ctypes.c_void_p),
+        ("random_fill", ctypes.c_int),
+        ("exclude", ctypes.c_void_p),
+        ("vol_z", ctypes.c_int),
+        ("method", ctypes.c_int),
+        ("sart_iters", ctypes.c_int),
+        ("sart_subsets", ctypes.c_int),
+        ("sart_start_tilt", ctypes.c_int),
+        ("sart_n_tilts", ctypes.c_int),
+        ("sart_relax", ctypes.c_float),
+        ("subset_order", ctypes.c_int),
+        ("positivity", ctypes.c_int),
+        ("storage", ctypes.c_int),
+        ("out_layout", ctypes.c_int),
+        ("fill_seed", ctypes.c_uint64),
+    ]
+
+
+class AtrError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Load libatr_recon.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+            "g.build()'` (make -C aretomo3_amd/csrc)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, sz, u64, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_float
+    pi = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "atr_series_desc_init": (None, [ctypes.POINTER(SeriesDesc)]),
+        "atr_plan_create": (i, [ctypes.POINTER(SeriesDesc), ctypes.POINTER(vp)]),
+        "atr_plan_get_volume_size": (i, [vp, pi, pi, pi]),
+        "atr_plan_device_bytes": (sz, [vp]),
+        "atr_estimate_device_bytes": (i, [ctypes.POINTER(SeriesDesc), ctypes.POINTER(sz)]),
+        "atr_execute": (i, [vp, vp, vp, i, vp]),
+        "atr_execute_host": (i, [vp, vp, vp]),
+        "atr_plan_set_profiling": (i, [vp, i]),
+        "atr_plan_kernel_stats": (i, [vp, i, pi, ctypes.POINTER(ctypes.c_double)]),
+        "atr_plan_reset_stats": (i, [vp]),
+        "atr_plan_destroy": (None, [vp]),
+        "atr_last_error_string": (ctypes.c_char_p, []),
+        "atr_version": (ctypes.c_char_p, []),
+        "atr_phantom_tilt_series": (i, [u64, i, i, i, vp, i, i, f, f, f, f, f, vp, vp, i]),
+        "atr_device_count": (i, [pi]),
+        "atr_set_device": (i, [i]),
+        "atr_malloc": (i, [ctypes.POINTER(vp), sz]),
+        "atr_free": (i, [vp]),
+        "atr_memcpy_h2d": (i, [vp, vp, sz]),
+        "atr_memcpy_d2h": (i, [vp, vp, sz]),
+        "atr_memset": (i, [vp, i, sz]),
+        "atr_synchronize": (i, []),
+        "atr_stream_create": (i, [ctypes.POINTER(vp)]),
+        "atr_stream_destroy": (i, [vp]),
+        "atr_stream_synchronize": (i, [vp]),
+        "atr_mem_info": (i, [ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+        "atr_event_create": (i, [ctypes.POINTER(vp)]),
+        "atr_event_destroy": (i, [vp]),
+        "atr_event_record": (i, [vp, vp]),
+        "atr_event_elapsed_ms": (i, [vp, vp, ctypes.POINTER(ctypes.c_float)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != ATR_OK:
+        msg = lib().atr_last_error_string()
+        raise AtrError(status, msg.decode() if msg else "")
+    return status

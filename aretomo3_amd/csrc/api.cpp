@@ -1,0 +1,643 @@
+// api.cpp -- plan construction, stream-ordered execution and the C ABI of
+// libatr_recon.so.
+//
+// Host control flow mirrors the reference operators:
+//   SART  Recon/CTomoSart.cpp:54-90   weight -> BP(all, relax 1) ->
+//         iters x subsets {FP, diff, BP(relax)} -> fill empty
+//   WBP   Recon/CTomoWbp.cpp:38-60    weight -> ramp filter -> 2x-grid BP ->
+//         Fourier crop
+// but every kernel processes every slice of the series at once (the
+// reference loops over y-slices on the host, Recon/CDoWbpRecon.cpp:69-80),
+// and all state stays on the device between kernels.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "atr_internal.h"
+
+struct atr_plan {
+    atr::Plan p;
+};
+
+namespace atr {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+atr_status fail(atr_status st, const std::string& msg)
+{
+    g_err = msg;
+    return st;
+}
+
+atr_status hip_fail(hipError_t e, const char* what)
+{
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? ATR_ERR_OUT_OF_MEMORY : ATR_ERR_HIP;
+}
+
+template <class T>
+static atr_status upload(T** dst, const std::vector<T>& src, size_t& bytes)
+{
+    *dst = nullptr;
+    if (src.empty()) return ATR_OK;
+    size_t b = sizeof(T) * src.size();
+    ATR_HIP(hipMalloc((void**)dst, b));
+    ATR_HIP(hipMemcpy(*dst, src.data(), b, hipMemcpyHostToDevice));
+    bytes += b;
+    return ATR_OK;
+}
+
+static atr_status dalloc(void** dst, size_t b, size_t& bytes)
+{
+    *dst = nullptr;
+    if (b == 0) return ATR_OK;
+    ATR_HIP(hipMalloc(dst, b));
+    bytes += b;
+    return ATR_OK;
+}
+
+static size_t storage_bytes(int storage) { return storage == ATR_STORAGE_F32 ? 4 : 2; }
+
+// Sizes, validation and the SART subset lists; no device work.
+static atr_status plan_sizes(const atr_series_desc* d, Plan& p)
+{
+    if (d == nullptr) return fail(ATR_ERR_INVALID_ARG, "descriptor is NULL");
+    p.desc = *d;
+    if (d->proj_x < 4 || d->proj_y < 1 || d->n_tilts < 1)
+        return fail(ATR_ERR_INVALID_ARG, "proj_x >= 4, proj_y >= 1 and n_tilts >= 1 required");
+    if (d->tilt_deg == nullptr) return fail(ATR_ERR_INVALID_ARG, "tilt_deg is NULL");
+    if (d->vol_z < 2) return fail(ATR_ERR_INVALID_ARG, "vol_z >= 2 required");
+    if (d->method != ATR_METHOD_WBP && d->method != ATR_METHOD_SART)
+        return fail(ATR_ERR_INVALID_ARG, "unknown method");
+    if (d->storage < ATR_STORAGE_F32 || d->storage > ATR_STORAGE_BF16)
+        return fail(ATR_ERR_INVALID_ARG, "unknown storage");
+    if (d->out_layout < ATR_LAYOUT_XZY || d->out_layout > ATR_LAYOUT_SLICE)
+        return fail(ATR_ERR_INVALID_ARG, "unknown output layout");
+    p.n = d->n_tilts;
+    p.in_x = d->proj_x;
+    p.in_y = d->proj_y;
+    p.in_row_stride = d->row_stride ? d->row_stride : (size_t)d->proj_x;
+    p.in_tilt_stride = d->tilt_stride ? d->tilt_stride : p.in_row_stride * d->proj_y;
+    if (p.in_row_stride < (size_t)d->proj_x ||
+        p.in_tilt_stride < p.in_row_stride * (d->proj_y - 1) + d->proj_x)
+        return fail(ATR_ERR_INVALID_ARG, "input strides smaller than the image");
+    for (int t = 0; t < p.n; t++)
+        if (!std::isfinite(d->tilt_deg[t])) return fail(ATR_ERR_INVALID_ARG, "non-finite tilt");
+    p.prestage = d->tilt_axis_deg != nullptr;
+    if (p.prestage) {
+        aligned_size(p.in_x, p.in_y, d->tilt_axis_deg[0], &p.aln_x, &p.aln_y);
+        for (int t = 0; t < p.n; t++)
+            if (!std::isfinite(d->tilt_axis_deg[t]) ||
+                (d->shift_xy && (!std::isfinite(d->shift_xy[2 * t]) ||
+                                 !std::isfinite(d->shift_xy[2 * t + 1]))))
+                return fail(ATR_ERR_INVALID_ARG, "non-finite tilt axis or shift");
+    } else {
+        p.aln_x = p.in_x;
+        p.aln_y = p.in_y;
+    }
+    p.fill = p.prestage && d->random_fill != 0;
+    // CDoWbpRecon.cpp:37-38, GBackProj.cu:74: VolX = ProjX/2*2 = effective ProjX
+    p.proj = p.aln_x / 2 * 2;
+    p.vol_x = p.proj;
+    p.vol_y = p.aln_y;
+    p.vol_z = d->vol_z;
+    p.sino_stride = (p.proj + 63) / 64 * 64;
+    p.method = d->method;
+    p.storage = d->storage;
+    p.layout = d->out_layout;
+
+    if (p.method == ATR_METHOD_SART) {
+        p.iters = d->sart_iters;
+        p.n_subsets = d->sart_subsets;
+        p.sart_start = d->sart_start_tilt;
+        p.sart_n = d->sart_n_tilts > 0 ? d->sart_n_tilts : p.n - p.sart_start;
+        p.order = d->subset_order;
+        p.positivity = d->positivity ? 1 : 0;
+        if (p.iters < 0) return fail(ATR_ERR_INVALID_ARG, "sart_iters < 0");
+        if (p.order != ATR_SUBSETS_CONTIGUOUS && p.order != ATR_SUBSETS_INTERLEAVED)
+            return fail(ATR_ERR_INVALID_ARG, "unknown subset order");
+        if (p.sart_start < 0 || p.sart_n < 1 || p.sart_start + p.sart_n > p.n)
+            return fail(ATR_ERR_INVALID_ARG, "SART tilt range outside the series");
+        if (p.n_subsets < 1 || p.n_subsets > p.sart_n)
+            return fail(ATR_ERR_INVALID_ARG, "sart_subsets must be in [1, number of SART tilts]");
+        p.relax = d->sart_relax > 0 ? d->sart_relax : 1.0f / (float)p.n_subsets;
+        // subset lists (Recon/CTomoSart.cpp:60-61,74-80; AreTomo/Util/CSplitItems.cpp)
+        std::vector<int> starts, sizes;
+        split_items(p.sart_n, p.n_subsets, starts, sizes);
+        p.subset_off.assign(1, 0);
+        p.subset_fp_off.assign(1, 0);
+        p.subset_tilts.clear();
+        p.subset_fp.clear();
+        p.max_subset = 0;
+        for (int s = 0; s < p.n_subsets; s++) {
+            std::vector<int> tl;
+            if (p.order == ATR_SUBSETS_CONTIGUOUS) {
+                for (int k = 0; k < sizes[s]; k++) tl.push_back(p.sart_start + starts[s] + k);
+            } else {
+                for (int k = s; k < p.sart_n; k += p.n_subsets) tl.push_back(p.sart_start + k);
+            }
+            for (int j = 0; j < (int)tl.size(); j++) {
+                p.subset_fp.push_back(tl[j]);
+                if (d->exclude && d->exclude[tl[j]]) continue;
+                p.subset_tilts.push_back(TiltRow{tl[j], j});
+            }
+            p.subset_off.push_back((int)p.subset_tilts.size());
+            p.subset_fp_off.push_back((int)p.subset_fp.size());
+            p.max_subset = std::max(p.max_subset, (int)tl.size());
+        }
+    }
+    p.all_tilts.clear();
+    for (int t = 0; t < p.n; t++)
+        if (!(d->exclude && d->exclude[t])) p.all_tilts.push_back(TiltRow{t, t});
+
+    if (p.method == ATR_METHOD_WBP) {
+        FftDesc f;
+        if (!make_fft(p.proj, f) || !make_fft(2 * p.vol_x, f) || !make_fft(2 * p.vol_z, f) ||
+            !make_fft(p.vol_z, f) || !make_fft(p.vol_x, f))
+            return fail(ATR_ERR_UNSUPPORTED_SIZE,
+                        "WBP FFT sizes must factor into 2, 3 and 5 (ProjX, VolX, VolZ)");
+        if (p.proj > 4096 || p.vol_x > 2048 || p.vol_z > 2048)
+            return fail(ATR_ERR_UNSUPPORTED_SIZE, "WBP supports ProjX <= 4096, VolZ <= 2048");
+        // slab of slices whose spectra are buffered between the WBP kernels
+        size_t per_slice = (size_t)3 * p.vol_z * (p.vol_x / 2 + 1) * sizeof(float2);
+        size_t budget = (size_t)1 << 30;
+        p.wbp_slab = (int)std::max<size_t>(1, std::min<size_t>(p.vol_y, budget / per_slice));
+    }
+    return ATR_OK;
+}
+
+static size_t plan_bytes_estimate(const Plan& p)
+{
+    size_t b = 0;
+    size_t n = p.n, sb = storage_bytes(p.storage);
+    b += n * 64 + (size_t)p.vol_y * 64;
+    if (p.prestage) {
+        b += n * p.aln_x * (size_t)p.aln_y * 4 + n * p.aln_y * 16 + (size_t)p.aln_x * p.aln_y * 4;
+    }
+    b += (size_t)p.vol_y * n * p.sino_stride * sb;
+    if (p.method == ATR_METHOD_SART) {
+        size_t vox = (size_t)p.vol_x * p.vol_y * p.vol_z;
+        b += vox * 4 + (size_t)p.vol_y * p.max_subset * p.sino_stride * sb;
+        b += n * p.vol_z * 8 + n * p.proj * 8 + (size_t)p.vol_x * p.vol_z * 4;
+        b += n * (size_t)(p.vol_z * 2.5 + 16) * 16;  // FP sample tables (bounded by 60 deg)
+    } else {
+        b += (size_t)p.wbp_slab * 3 * p.vol_z * (p.vol_x / 2 + 1) * sizeof(float2);
+        b += n * 2 * p.vol_z * 8;
+    }
+    return b;
+}
+
+static atr_status plan_build(Plan& p)
+{
+    const atr_series_desc& d = p.desc;
+    Geometry& g = p.geo;
+    cos_sin_table(d.tilt_deg, p.n, g.cos_sin);
+    size_t& B = p.device_bytes;
+    B = 0;
+    std::vector<float2> cs(p.n);
+    for (int t = 0; t < p.n; t++) cs[t] = make_float2(g.cos_sin[2 * t], g.cos_sin[2 * t + 1]);
+    atr_status st;
+    if ((st = upload(&p.d_cs, cs, B)) != ATR_OK) return st;
+    std::vector<unsigned char> excl(p.n, 0);
+    if (d.exclude)
+        for (int t = 0; t < p.n; t++) excl[t] = d.exclude[t] ? 1 : 0;
+    if ((st = upload(&p.d_excl, excl, B)) != ATR_OK) return st;
+
+    if (p.prestage) {
+        // Correct/GCorrPatchShift.cu:104,142: angle = axis * (3.141592654f/180)
+        // in float, cos/sin in double.
+        const float d2r = 3.141592654f / 180.0f;
+        std::vector<double> acs(2 * (size_t)p.n);
+        std::vector<float> sh(2 * (size_t)p.n, 0.0f);
+        for (int t = 0; t < p.n; t++) {
+            float ang = d.tilt_axis_deg[t] * d2r;
+            acs[2 * t] = std::cos((double)ang);
+            acs[2 * t + 1] = std::sin((double)ang);
+            if (d.shift_xy) {
+                sh[2 * t] = d.shift_xy[2 * t];
+                sh[2 * t + 1] = d.shift_xy[2 * t + 1];
+            }
+        }
+        if ((st = upload(&p.d_axis_cs, acs, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_shifts, sh, B)) != ATR_OK) return st;
+        if ((st = dalloc((void**)&p.d_aligned, sizeof(float) * p.n * (size_t)p.aln_x * p.aln_y, B)) != ATR_OK) return st;
+        if ((st = dalloc((void**)&p.d_row_stats, sizeof(double) * 2 * p.n * (size_t)p.aln_y, B)) != ATR_OK) return st;
+        if ((st = dalloc((void**)&p.d_tilt_stats, sizeof(double) * 2 * p.n, B)) != ATR_OK) return st;
+        if (p.fill) {
+            std::vector<int> tbl;
+            fill_table(d.fill_seed, p.aln_x * p.aln_y, tbl);
+            if ((st = upload(&p.d_fill_aln, tbl, B)) != ATR_OK) return st;
+        }
+    }
+    size_t sb = storage_bytes(p.storage);
+    if ((st = dalloc(&p.d_sino, (size_t)p.vol_y * p.n * p.sino_stride * sb, B)) != ATR_OK) return st;
+    if ((st = dalloc((void**)&p.d_sflag, sizeof(int) * p.vol_y, B)) != ATR_OK) return st;
+
+    if (p.method == ATR_METHOD_SART) {
+        build_bp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.bp_lohi);
+        build_bp_classes(g.bp_lohi, p.n, p.vol_x, p.vol_z, g.bp_class);
+        if (!build_fp_tables(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g))
+            return fail(ATR_ERR_INVALID_ARG, "SART needs every tilt in (-90, 90) degrees");
+        if ((st = upload(&p.d_bp_lohi, g.bp_lohi, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_bp_class, g.bp_class, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_fp_range, g.fp_range, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_fp_tile, g.fp_tile, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_fp_geo_off, g.fp_geo_off, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_fp_len, g.fp_len, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_fp_geo, g.fp_geo, B)) != ATR_OK) return st;
+        std::vector<TiltRow> lists(p.all_tilts);
+        lists.insert(lists.end(), p.subset_tilts.begin(), p.subset_tilts.end());
+        if ((st = upload(&p.d_lists, lists, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_fp_lists, p.subset_fp, B)) != ATR_OK) return st;
+        std::vector<int> tbl;
+        fill_table(d.fill_seed ^ 0x5A5A5A5A5A5A5A5Aull, p.vol_x * p.vol_z, tbl);
+        if ((st = upload(&p.d_fill_vol, tbl, B)) != ATR_OK) return st;
+        size_t vox = (size_t)p.vol_x * p.vol_y * p.vol_z;
+        if ((st = dalloc((void**)&p.d_vol, sizeof(float) * vox, B)) != ATR_OK) return st;
+        if ((st = dalloc(&p.d_diff, (size_t)p.vol_y * p.max_subset * p.sino_stride * sb, B)) != ATR_OK) return st;
+        if ((st = dalloc((void**)&p.d_vflag, sizeof(int) * p.vol_y, B)) != ATR_OK) return st;
+        if ((st = dalloc((void**)&p.d_vstats, sizeof(double) * 2 * p.vol_y, B)) != ATR_OK) return st;
+    } else {
+        build_wbp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.wbp_lohi);
+        if ((st = upload(&p.d_wbp_lohi, g.wbp_lohi, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_filter, ramp_filter(p.proj), B)) != ATR_OK) return st;
+        make_fft(p.proj, p.fft_proj);
+        make_fft(2 * p.vol_x, p.fft_x2);
+        make_fft(2 * p.vol_z, p.fft_z2);
+        make_fft(p.vol_z, p.fft_z);
+        make_fft(p.vol_x, p.fft_x);
+        if ((st = upload(&p.d_tw_proj, twiddles(p.proj), B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_tw_x2, twiddles(2 * p.vol_x), B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_tw_z2, twiddles(2 * p.vol_z), B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_tw_z, twiddles(p.vol_z), B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_tw_x, twiddles(p.vol_x), B)) != ATR_OK) return st;
+        p.fft_proj.tw = p.d_tw_proj;
+        p.fft_x2.tw = p.d_tw_x2;
+        p.fft_z2.tw = p.d_tw_z2;
+        p.fft_z.tw = p.d_tw_z;
+        p.fft_x.tw = p.d_tw_x;
+        size_t kxn = p.vol_x / 2 + 1;
+        if ((st = dalloc((void**)&p.d_wbp_t, (size_t)p.wbp_slab * 2 * p.vol_z * kxn * sizeof(float2), B)) != ATR_OK) return st;
+        if ((st = dalloc((void**)&p.d_wbp_u, (size_t)p.wbp_slab * p.vol_z * kxn * sizeof(float2), B)) != ATR_OK) return st;
+    }
+    return ATR_OK;
+}
+
+static void plan_free(Plan& p)
+{
+    void* ptrs[] = {p.d_cs, p.d_bp_lohi, p.d_bp_class, p.d_wbp_lohi, p.d_fp_range, p.d_fp_tile,
+                    p.d_fp_geo_off, p.d_fp_len, p.d_fp_geo, p.d_lists, p.d_fp_lists,
+                    p.d_fill_aln, p.d_fill_vol, p.d_filter, p.d_tw_proj, p.d_tw_x2, p.d_tw_z2,
+                    p.d_tw_z, p.d_tw_x, p.d_axis_cs, p.d_shifts, p.d_excl, p.d_aligned,
+                    p.d_row_stats, p.d_tilt_stats, p.d_sino, p.d_diff, p.d_vol, p.d_sflag,
+                    p.d_vflag, p.d_vstats, p.d_wbp_t, p.d_wbp_u};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    for (auto& e : p.ev_pool) (void)hipEventDestroy(e);
+    for (auto& e : p.ev_pending) {
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+    }
+    p.ev_pool.clear();
+    p.ev_pending.clear();
+}
+
+// ---- profiling -----------------------------------------------------------
+static hipEvent_t take_event(Plan& p)
+{
+    if (!p.ev_pool.empty()) {
+        hipEvent_t e = p.ev_pool.back();
+        p.ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+static void fold_events(Plan& p)
+{
+    for (auto& e : p.ev_pending) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(e.second.second) == hipSuccess &&
+            hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess) {
+            p.launches[e.first] += 1;
+            p.total_ms[e.first] += ms;
+        }
+        p.ev_pool.push_back(e.second.first);
+        p.ev_pool.push_back(e.second.second);
+    }
+    p.ev_pending.clear();
+}
+
+template <class F>
+static void timed(Plan& p, int kernel, hipStream_t s, F&& fn)
+{
+    if (!p.profiling) {
+        fn();
+        return;
+    }
+    hipEvent_t a = take_event(p), b = take_event(p);
+    if (a == nullptr || b == nullptr) {
+        fn();
+        return;
+    }
+    (void)hipEventRecord(a, s);
+    fn();
+    (void)hipEventRecord(b, s);
+    p.ev_pending.push_back({kernel, {a, b}});
+    if (p.ev_pending.size() > 4096) fold_events(p);
+}
+
+// ---- execution -----------------------------------------------------------
+static atr_status plan_execute(Plan& p, const float* in, float* out, int accumulate, hipStream_t s)
+{
+    if (in == nullptr || out == nullptr) return fail(ATR_ERR_INVALID_ARG, "NULL buffer");
+    (void)hipGetLastError();
+    if (p.prestage) {
+        timed(p, K_PRESTAGE, s, [&] {
+            launch_prestage(p, in, s);
+            launch_tilt_stats(p, s);
+        });
+    }
+    launch_zero_int(p.d_sflag, p.vol_y, s);
+    if (p.method == ATR_METHOD_SART) {
+        launch_zero_int(p.d_vflag, p.vol_y, s);
+        timed(p, K_WEIGHT, s, [&] { launch_weight_sart(p, in, s); });
+        if (accumulate && p.layout == ATR_LAYOUT_SLICE) launch_load_volume(p, out, s);
+        else launch_zero(p.d_vol, (size_t)p.vol_x * p.vol_y * p.vol_z, s);
+        const TiltRow* d_all = p.d_lists;
+        const int n_all = (int)p.all_tilts.size();
+        // initial back-projection of every projection, relax 1 (CTomoSart.cpp:68-69)
+        if (n_all > 0)
+            timed(p, K_BP, s, [&] { launch_bp(p, p.d_sino, p.n, d_all, n_all, 1.0f, true, s); });
+        for (int it = 0; it < p.iters; it++) {
+            for (int sb = 0; sb < p.n_subsets; sb++) {
+                int f0 = p.subset_fp_off[sb], f1 = p.subset_fp_off[sb + 1];
+                int b0 = p.subset_off[sb], b1 = p.subset_off[sb + 1];
+                timed(p, K_FP, s, [&] { launch_fp_diff(p, p.d_fp_lists + f0, f1 - f0, s); });
+                if (b1 > b0)
+                    timed(p, K_BP, s, [&] {
+                        launch_bp(p, p.d_diff, p.max_subset, d_all + n_all + b0, b1 - b0,
+                                  p.relax, false, s);
+                    });
+            }
+        }
+        timed(p, K_OUTPUT, s, [&] {
+            launch_vol_stats(p, s);
+            launch_output(p, out, s);
+        });
+    } else {
+        timed(p, K_WEIGHT, s, [&] { launch_filter_wbp(p, in, s); });
+        for (int y0 = 0; y0 < p.vol_y; y0 += p.wbp_slab) {
+            int ny = std::min(p.wbp_slab, p.vol_y - y0);
+            timed(p, K_WBP_BP, s, [&] { launch_wbp_bp_rowfft(p, y0, ny, s); });
+            timed(p, K_WBP_COL, s, [&] { launch_wbp_colfft(p, y0, ny, s); });
+            timed(p, K_WBP_ROW, s, [&] { launch_wbp_rowc2r(p, y0, ny, out, s); });
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return ATR_OK;
+}
+
+}  // namespace atr
+
+// ============================================================================
+// C ABI
+// ============================================================================
+using namespace atr;
+
+extern "C" {
+
+void atr_series_desc_init(atr_series_desc* d)
+{
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    d->random_fill = 1;
+    d->vol_z = 100;
+    d->method = ATR_METHOD_SART;
+    d->sart_iters = 20;   // -Sart 20 3 (CAtInput.cpp:69-70)
+    d->sart_subsets = 1;
+    d->sart_relax = -1.0f;
+    d->subset_order = ATR_SUBSETS_CONTIGUOUS;
+    d->positivity = 1;
+    d->storage = ATR_STORAGE_F32;
+    d->out_layout = ATR_LAYOUT_XZY;
+}
+
+atr_status atr_plan_create(const atr_series_desc* desc, atr_plan** plan)
+{
+    if (plan == nullptr) return fail(ATR_ERR_INVALID_ARG, "plan is NULL");
+    *plan = nullptr;
+    atr_plan* h = new (std::nothrow) atr_plan();
+    if (!h) return fail(ATR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    atr_status st = plan_sizes(desc, h->p);
+    if (st == ATR_OK) st = plan_build(h->p);
+    if (st != ATR_OK) {
+        plan_free(h->p);
+        delete h;
+        return st;
+    }
+    *plan = h;
+    return ATR_OK;
+}
+
+atr_status atr_estimate_device_bytes(const atr_series_desc* desc, size_t* bytes)
+{
+    if (bytes == nullptr) return fail(ATR_ERR_INVALID_ARG, "bytes is NULL");
+    Plan p;
+    atr_status st = plan_sizes(desc, p);
+    if (st != ATR_OK) return st;
+    *bytes = plan_bytes_estimate(p);
+    return ATR_OK;
+}
+
+atr_status atr_plan_get_volume_size(const atr_plan* plan, int* vx, int* vy, int* vz)
+{
+    if (!plan) return fail(ATR_ERR_INVALID_ARG, "plan is NULL");
+    if (vx) *vx = plan->p.vol_x;
+    if (vy) *vy = plan->p.vol_y;
+    if (vz) *vz = plan->p.vol_z;
+    return ATR_OK;
+}
+
+size_t atr_plan_device_bytes(const atr_plan* plan) { return plan ? plan->p.device_bytes : 0; }
+
+atr_status atr_execute(atr_plan* plan, const float* proj_dev, float* vol_dev, int accumulate,
+                       void* stream)
+{
+    if (!plan) return fail(ATR_ERR_INVALID_ARG, "plan is NULL");
+    return plan_execute(plan->p, proj_dev, vol_dev, accumulate, (hipStream_t)stream);
+}
+
+atr_status atr_execute_host(atr_plan* plan, const float* proj_host, float* vol_host)
+{
+    if (!plan || !proj_host || !vol_host) return fail(ATR_ERR_INVALID_ARG, "NULL argument");
+    Plan& p = plan->p;
+    size_t in_elems = p.in_tilt_stride * (p.n - 1) + p.in_row_stride * (p.in_y - 1) + p.in_x;
+    size_t out_elems = (size_t)p.vol_x * p.vol_y * p.vol_z;
+    float *d_in = nullptr, *d_out = nullptr;
+    hipStream_t s = nullptr;
+    atr_status st = ATR_OK;
+    hipError_t e = hipMalloc((void**)&d_in, in_elems * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&d_out, out_elems * sizeof(float));
+    if (e == hipSuccess) e = hipStreamCreate(&s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_in, proj_host, in_elems * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        st = plan_execute(p, d_in, d_out, 0, s);
+        if (st == ATR_OK)
+            e = hipMemcpyAsync(vol_host, d_out, out_elems * sizeof(float), hipMemcpyDeviceToHost, s);
+        if (st == ATR_OK && e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (e != hipSuccess) st = hip_fail(e, "atr_execute_host");
+    if (s) (void)hipStreamDestroy(s);
+    if (d_in) (void)hipFree(d_in);
+    if (d_out) (void)hipFree(d_out);
+    return st;
+}
+
+atr_status atr_plan_set_profiling(atr_plan* plan, int enable)
+{
+    if (!plan) return fail(ATR_ERR_INVALID_ARG, "plan is NULL");
+    plan->p.profiling = enable != 0;
+    return ATR_OK;
+}
+
+atr_status atr_plan_kernel_stats(const atr_plan* plan, int kernel, int* launches, double* total_ms)
+{
+    if (!plan || kernel < 0 || kernel >= K_COUNT) return fail(ATR_ERR_INVALID_ARG, "bad argument");
+    Plan& p = const_cast<atr_plan*>(plan)->p;
+    fold_events(p);
+    if (launches) *launches = p.launches[kernel];
+    if (total_ms) *total_ms = p.total_ms[kernel];
+    return ATR_OK;
+}
+
+atr_status atr_plan_reset_stats(atr_plan* plan)
+{
+    if (!plan) return fail(ATR_ERR_INVALID_ARG, "plan is NULL");
+    fold_events(plan->p);
+    for (int k = 0; k < K_COUNT; k++) {
+        plan->p.launches[k] = 0;
+        plan->p.total_ms[k] = 0.0;
+    }
+    return ATR_OK;
+}
+
+void atr_plan_destroy(atr_plan* plan)
+{
+    if (!plan) return;
+    (void)hipDeviceSynchronize();
+    plan_free(plan->p);
+    delete plan;
+}
+
+const char* atr_last_error_string(void) { return g_err.c_str(); }
+
+const char* atr_version(void) { return "atr_recon 0.1 (gfx950)"; }
+
+// ---- device helpers --------------------------------------------------------
+atr_status atr_device_count(int* count)
+{
+    if (!count) return fail(ATR_ERR_INVALID_ARG, "count is NULL");
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    return ATR_OK;
+}
+atr_status atr_set_device(int device)
+{
+    ATR_HIP(hipSetDevice(device));
+    return ATR_OK;
+}
+atr_status atr_malloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(ATR_ERR_INVALID_ARG, "ptr is NULL");
+    ATR_HIP(hipMalloc(ptr, bytes));
+    return ATR_OK;
+}
+atr_status atr_free(void* ptr)
+{
+    ATR_HIP(hipFree(ptr));
+    return ATR_OK;
+}
+atr_status atr_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    ATR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return ATR_OK;
+}
+atr_status atr_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    ATR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return ATR_OK;
+}
+atr_status atr_memset(void* ptr, int value, size_t bytes)
+{
+    ATR_HIP(hipMemset(ptr, value, bytes));
+    return ATR_OK;
+}
+atr_status atr_synchronize(void)
+{
+    ATR_HIP(hipDeviceSynchronize());
+    return ATR_OK;
+}
+atr_status atr_stream_create(void** stream)
+{
+    if (!stream) return fail(ATR_ERR_INVALID_ARG, "stream is NULL");
+    hipStream_t s = nullptr;
+    ATR_HIP(hipStreamCreate(&s));
+    *stream = s;
+    return ATR_OK;
+}
+atr_status atr_stream_destroy(void* stream)
+{
+    ATR_HIP(hipStreamDestroy((hipStream_t)stream));
+    return ATR_OK;
+}
+atr_status atr_stream_synchronize(void* stream)
+{
+    ATR_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return ATR_OK;
+}
+atr_status atr_mem_info(size_t* free_bytes, size_t* total_bytes)
+{
+    size_t f = 0, t = 0;
+    ATR_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return ATR_OK;
+}
+atr_status atr_event_create(void** ev)
+{
+    if (!ev) return fail(ATR_ERR_INVALID_ARG, "ev is NULL");
+    hipEvent_t e = nullptr;
+    ATR_HIP(hipEventCreate(&e));
+    *ev = e;
+    return ATR_OK;
+}
+atr_status atr_event_destroy(void* ev)
+{
+    ATR_HIP(hipEventDestroy((hipEvent_t)ev));
+    return ATR_OK;
+}
+atr_status atr_event_record(void* ev, void* stream)
+{
+    ATR_HIP(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return ATR_OK;
+}
+atr_status atr_event_elapsed_ms(void* a, void* b, float* ms)
+{
+    if (!ms) return fail(ATR_ERR_INVALID_ARG, "ms is NULL");
+    ATR_HIP(hipEventSynchronize((hipEvent_t)b));
+    ATR_HIP(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return ATR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// atr_internal.h -- plan structure, geometry tables and kernel launchers of
+// libatr_recon.so (MI355X / gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/aretomo_recon.h"
+#include "fft_lds.h"
+
+namespace atr {
+
+// Sentinels of the reference (GBackProj.cu:45,55-59; GForProj.cu:161,181).
+constexpr float kEmpty = -1e10f;
+constexpr float kSentinel = -1e30f;
+
+// ---- tile shapes ---------------------------------------------------------
+// SART back-projection: a workgroup owns BP_TX x BP_TZ voxels of one slice;
+// lane = x, each thread walks BP_TZ rows of z.
+constexpr int BP_TX = 256;
+constexpr int BP_TZ = 16;
+constexpr int BP_CHUNK = 8;                  // tilts staged in LDS at once
+constexpr int BP_WIN = BP_TX + BP_TZ + 16;   // staged pairs per tilt
+// Forward projection: one ray per lane, FP_RAYS rays per workgroup.
+constexpr int FP_RAYS = 256;
+constexpr int FP_WAVE = 64;
+
+enum Kernel {
+    K_BP = 0,
+    K_FP = 1,
+    K_WEIGHT = 2,
+    K_WBP_BP = 3,
+    K_PRESTAGE = 4,
+    K_OUTPUT = 5,
+    K_WBP_COL = 6,
+    K_WBP_ROW = 7,
+    K_COUNT = 8
+};
+
+// Per-slice SART tilt list entry: tilt index and the row of the input
+// sinogram/diff buffer that holds it.
+struct TiltRow {
+    int tilt;
+    int row;
+};
+
+// FP64 geometry evaluated once per plan on the host.
+struct Geometry {
+    std::vector<float> cos_sin;        // [2N] float, as CTomoBase.cpp:55-65
+    std::vector<int2> bp_lohi;         // [N][VolZ] x-interval of GBackProj inclusion
+    std::vector<uint8_t> bp_class;     // [N][ztiles][xtiles] 0 empty, 1 full, 2 partial
+    std::vector<int2> wbp_lohi;        // [N][2VolZ] x2-interval of GBackProjWbp inclusion
+    std::vector<int2> fp_range;        // [N][P] valid sample interval of each ray
+    std::vector<int4> fp_tile;         // [N][P/64] out_lo, core_lo, core_hi, out_hi
+    std::vector<int> fp_len;           // [N] ray length L
+    std::vector<int> fp_geo_off;       // [N] offset into fp_geo
+    std::vector<float4> fp_geo;        // [sum L] {row offset bits, wz, q, 0}
+};
+
+struct Plan {
+    atr_series_desc desc{};
+    // sizes
+    int n = 0;            // tilts
+    int in_x = 0, in_y = 0;   // input image size
+    size_t in_tilt_stride = 0, in_row_stride = 0;
+    int aln_x = 0, aln_y = 0; // aligned image (= input when no pre-stage)
+    int proj = 0;         // effective projection width (ProjX/2*2)
+    int vol_x = 0, vol_y = 0, vol_z = 0;
+    int sino_stride = 0;  // elements per sinogram row on the device
+    bool prestage = false;
+    bool fill = false;
+    int method = 0, storage = 0, layout = 0;
+    // SART
+    int iters = 0, n_subsets = 1, sart_start = 0, sart_n = 0, order = 0, positivity = 1;
+    float relax = 1.0f;
+    int max_subset = 0;
+    std::vector<int> subset_off;      // [n_subsets + 1] offsets into subset_tilts
+    std::vector<TiltRow> subset_tilts;// BP lists (excluded tilts removed)
+    std::vector<int> subset_fp;       // FP tilt lists (all tilts of the subset)
+    std::vector<int> subset_fp_off;
+    std::vector<TiltRow> all_tilts;   // initial BP list
+    // geometry
+    Geometry geo;
+    // device tables
+    float2* d_cs = nullptr;
+    int2* d_bp_lohi = nullptr;
+    uint8_t* d_bp_class = nullptr;
+    int2* d_wbp_lohi = nullptr;
+    int2* d_fp_range = nullptr;
+    int4* d_fp_tile = nullptr;
+    int* d_fp_geo_off = nullptr;
+    int* d_fp_len = nullptr;
+    float4* d_fp_geo = nullptr;
+    TiltRow* d_lists = nullptr;       // all_tilts then every subset's BP list
+    int* d_fp_lists = nullptr;        // every subset's FP list
+    int* d_fill_aln = nullptr;        // [aln_x*aln_y]
+    int* d_fill_vol = nullptr;        // [vol_x*vol_z]
+    float* d_filter = nullptr;        // [proj/2+1] ramp/Hamming weights
+    float2* d_tw_proj = nullptr;      // twiddles for the row filter
+    float2* d_tw_x2 = nullptr;        // 2 VolX
+    float2* d_tw_z2 = nullptr;        // 2 VolZ
+    float2* d_tw_z = nullptr;         // VolZ
+    float2* d_tw_x = nullptr;         // VolX
+    // device work buffers
+    double* d_axis_cs = nullptr;      // [N][2] cos/sin of the tilt axis (FP64)
+    float* d_shifts = nullptr;        // [N][2] global shifts
+    unsigned char* d_excl = nullptr;  // [N] excluded tilts
+    float* d_aligned = nullptr;       // [N][aln_y][aln_x] pre-stage output with sentinels
+    double* d_row_stats = nullptr;    // [N][aln_y][2] per-row (sum, count) of non-empty pixels
+    double* d_tilt_stats = nullptr;   // [N][2] (sum, count) of non-empty aligned pixels
+    void* d_sino = nullptr;           // [vol_y][N][sino_stride] weighted projections
+    void* d_diff = nullptr;           // [vol_y][max_subset][sino_stride]
+    float* d_vol = nullptr;           // [vol_y][vol_z][vol_x] SART state
+    int* d_sflag = nullptr;           // [vol_y] sinogram slice holds sentinels
+    int* d_vflag = nullptr;           // [vol_y] volume slice holds sentinels
+    double* d_vstats = nullptr;       // [vol_y][2] fill statistics of volume slices
+    float2* d_wbp_t = nullptr;        // [slab][2VolZ][VolX/2+1]
+    float2* d_wbp_u = nullptr;        // [slab][VolZ][VolX/2+1]
+    int wbp_slab = 0;
+    size_t device_bytes = 0;
+    // FFT descriptors
+    FftDesc fft_proj{}, fft_x2{}, fft_z2{}, fft_z{}, fft_x{};
+    // profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    int launches[K_COUNT] = {0};
+    double total_ms[K_COUNT] = {0};
+    hipStream_t stream = nullptr;
+};
+
+// ---- error handling ------------------------------------------------------
+void set_error(const std::string& msg);
+atr_status fail(atr_status st, const std::string& msg);
+atr_status hip_fail(hipError_t e, const char* what);
+#define ATR_HIP(call)                                         \
+    do {                                                      \
+        hipError_t _e = (call);                               \
+        if (_e != hipSuccess) return ::atr::hip_fail(_e, #call); \
+    } while (0)
+
+// ---- host geometry (geometry.cpp) ---------------------------------------
+void cos_sin_table(const float* tilts, int n, std::vector<float>& cs);
+void build_bp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
+                        std::vector<int2>& lohi);
+void build_bp_classes(const std::vector<int2>& lohi, int n, int vol_x, int vol_z,
+                      std::vector<uint8_t>& cls);
+void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
+                         std::vector<int2>& lohi);
+bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
+                     Geometry& g);
+void split_items(int n_items, int n_splits, std::vector<int>& starts, std::vector<int>& sizes);
+void fill_table(uint64_t seed, int n, std::vector<int>& table);
+void aligned_size(int raw_x, int raw_y, float axis_deg, int* ax, int* ay);
+bool make_fft(int n, FftDesc& fd);
+std::vector<float2> twiddles(int n);
+std::vector<float> ramp_filter(int n);
+uint64_t splitmix64(uint64_t x);
+
+// ---- kernel launchers (kernels_*.hip) -----------------------------------
+void launch_prestage(const Plan& p, const float* in, hipStream_t s);
+void launch_tilt_stats(const Plan& p, hipStream_t s);
+void launch_weight_sart(const Plan& p, const float* in, hipStream_t s);
+void launch_filter_wbp(const Plan& p, const float* in, hipStream_t s);
+void launch_zero(float* ptr, size_t n, hipStream_t s);
+void launch_zero_int(int* ptr, int n, hipStream_t s);
+int wbp_col_block(int vol_z);
+void launch_bp(const Plan& p, const void* sino, int rows_per_slice, const TiltRow* list, int n_list,
+               float relax, bool check_sent, hipStream_t s);
+void launch_fp_diff(const Plan& p, const int* fp_list, int n_list, hipStream_t s);
+void launch_vol_stats(const Plan& p, hipStream_t s);
+void launch_output(const Plan& p, float* out, hipStream_t s);
+void launch_wbp_bp_rowfft(const Plan& p, int y0, int ny, hipStream_t s);
+void launch_wbp_colfft(const Plan& p, int y0, int ny, hipStream_t s);
+void launch_wbp_rowc2r(const Plan& p, int y0, int ny, float* out, hipStream_t s);
+void launch_load_volume(const Plan& p, const float* in, hipStream_t s);
+
+}  // namespace atr

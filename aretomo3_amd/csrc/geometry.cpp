@@ -1,0 +1,300 @@
+// geometry.cpp -- per-plan geometry evaluated once on the host in FP64.
+//
+// The reference decides, per voxel and tilt, whether a projection sample is
+// used with FP32 tests such as `fXp < 0 || fXp > iProjEndX`
+// (Recon/GBackProj.cu:35-37, Recon/GBackProjWbp.cu:34-36,
+// Recon/GForProj.cu:145-149).  Those decisions are the only discontinuous
+// part of the method, so this build evaluates them in double, once per plan,
+// and hands the kernels integer intervals: per (tilt, z) the x-range a
+// back-projection row uses, per (tilt, detector x) the sample range a
+// forward-projection ray uses.  Inside the intervals the kernels interpolate
+// in FP32.  The expressions are written exactly as the FP64 restatement of
+// the reference evaluates them, and this file is compiled with
+// -ffp-contract=off, so every inclusion decision is reproducible bit for bit.
+#include <algorithm>
+#include <cmath>
+#include <thread>
+
+#include "atr_internal.h"
+
+namespace atr {
+
+uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Seeded stand-in for the cuRAND table of MaUtil/GGenRandoms.cu:7-21 (which
+// the reference seeds with time(0)): tbl_i = ((r_i >> 32) * n) >> 32 with
+// r_i = splitmix64(seed + i * 0xD1B54A32D192ED03).
+void fill_table(uint64_t seed, int n, std::vector<int>& table)
+{
+    table.resize(n);
+    for (int i = 0; i < n; i++) {
+        uint64_t r = splitmix64(seed + (uint64_t)i * 0xD1B54A32D192ED03ull);
+        table[i] = (int)(((r >> 32) * (uint64_t)n) >> 32);
+    }
+}
+
+// Recon/CTomoBase.cpp:55-65.
+void cos_sin_table(const float* tilts, int n, std::vector<float>& cs)
+{
+    const float rad = 0.017453f;
+    cs.resize(2 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        float ang = rad * tilts[i];
+        cs[2 * i] = (float)std::cos((double)ang);
+        cs[2 * i + 1] = (float)std::sin((double)ang);
+    }
+}
+
+// AreTomo/Util/CSplitItems.cpp:25-48.
+void split_items(int n_items, int n_splits, std::vector<int>& starts, std::vector<int>& sizes)
+{
+    starts.assign(n_splits, 0);
+    sizes.assign(n_splits, n_items / n_splits);
+    for (int i = 0; i < n_items % n_splits; i++) sizes[i] += 1;
+    for (int i = 1; i < n_splits; i++) starts[i] = starts[i - 1] + sizes[i - 1];
+}
+
+// Correct/CCorrectUtil.cpp:11-21.
+void aligned_size(int raw_x, int raw_y, float axis_deg, int* ax, int* ay)
+{
+    double rot = std::fabs(std::sin(axis_deg * 3.14 / 180.0));
+    *ax = raw_x;
+    *ay = raw_y;
+    if (rot <= 0.707) return;
+    *ax = raw_y;
+    *ay = raw_x;
+}
+
+// Interval {lo, hi} of integer i in [0, n) where pred(i) holds, for a
+// predicate that holds on one contiguous run (monotone geometry).  Empty
+// interval = {n, -1}.
+template <class P>
+static int2 run_of(int n, P pred)
+{
+    int lo = -1;
+    for (int i = 0; i < n; i++)
+        if (pred(i)) {
+            lo = i;
+            break;
+        }
+    if (lo < 0) return make_int2(n, -1);
+    int hi = lo;
+    while (hi + 1 < n && pred(hi + 1)) hi++;
+    return make_int2(lo, hi);
+}
+
+// Interval search on a monotone run: binary search for the two edges around a
+// known inside point would be faster, but the plain scan is already cheap
+// (VolX * VolZ * N evaluations) and is obviously exact.  It is split over
+// threads by tilt.
+template <class F>
+static void parallel_tilts(int n, F fn)
+{
+    unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < hw; w++)
+        th.emplace_back([=]() {
+            for (int t = (int)w; t < n; t += (int)hw) fn(t);
+        });
+    for (auto& x : th) x.join();
+}
+
+// Recon/GBackProj.cu:25-37: u = fx cos + fz sin + ProjX/2, used iff
+// 0 <= u <= ProjX-2, fx = x - VolX*0.5, fz = z - VolZ*0.5.
+void build_bp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
+                        std::vector<int2>& lohi)
+{
+    lohi.assign((size_t)n * vol_z, make_int2(vol_x, -1));
+    const double cx = proj / 2.0;
+    const int end_x = proj - 2;
+    parallel_tilts(n, [&](int t) {
+        double c = cs[2 * t], s = cs[2 * t + 1];
+        for (int z = 0; z < vol_z; z++) {
+            double fz = z - vol_z * 0.5;
+            lohi[(size_t)t * vol_z + z] = run_of(vol_x, [&](int x) {
+                double fx = x - vol_x * 0.5;
+                double u = fx * c + fz * s + cx;
+                return !(u < 0 || u > end_x);
+            });
+        }
+    });
+}
+
+// Tile class of every (tilt, z-tile, x-tile) of the SART back-projector:
+// 1 = every voxel of the tile uses the tilt, 0 = none does, 2 = some do.
+void build_bp_classes(const std::vector<int2>& lohi, int n, int vol_x, int vol_z,
+                      std::vector<uint8_t>& cls)
+{
+    int nzt = (vol_z + BP_TZ - 1) / BP_TZ, nxt = (vol_x + BP_TX - 1) / BP_TX;
+    cls.assign((size_t)n * nzt * nxt, 0);
+    for (int t = 0; t < n; t++)
+        for (int zt = 0; zt < nzt; zt++)
+            for (int xt = 0; xt < nxt; xt++) {
+                int x0 = xt * BP_TX, x1 = std::min(x0 + BP_TX, vol_x) - 1;
+                bool full = true, any = false;
+                for (int z = zt * BP_TZ; z < std::min((zt + 1) * BP_TZ, vol_z); z++) {
+                    int2 r = lohi[(size_t)t * vol_z + z];
+                    if (!(r.x <= x0 && r.y >= x1)) full = false;
+                    if (!(r.x > x1 || r.y < x0)) any = true;
+                }
+                cls[((size_t)t * nzt + zt) * nxt + xt] = full ? 1 : (any ? 2 : 0);
+            }
+}
+
+// Recon/GBackProjWbp.cu:23-36 on the 2x grid (vol_x2 = 2 VolX, vol_z2 =
+// 2 VolZ): u = (fx cos + fz sin)/2 + (ProjX/2) + 0.5, used iff 0 <= u <= ProjX-2.
+void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
+                         std::vector<int2>& lohi)
+{
+    const int vx2 = 2 * vol_x, vz2 = 2 * vol_z;
+    lohi.assign((size_t)n * vz2, make_int2(vx2, -1));
+    const int cx = proj / 2;
+    const int end_x = proj - 2;
+    parallel_tilts(n, [&](int t) {
+        double c = cs[2 * t], s = cs[2 * t + 1];
+        for (int z = 0; z < vz2; z++) {
+            double fz = z - vz2 / 2.0;
+            lohi[(size_t)t * vz2 + z] = run_of(vx2, [&](int x) {
+                double fx = x - vx2 / 2.0;
+                double u = (fx * c + fz * s) / 2 + cx + 0.5;
+                return !(u < 0 || u > end_x);
+            });
+        }
+    });
+}
+
+// Recon/GForProj.cu:131-149: ray (t, xd) has L = (int)(VolZ/cos + 1.5)
+// samples; sample k at fzk = k - L*0.5, x = ((xd - 0.5 ProjX)/cos + VolX*0.5)
+// - fzk sin, z = VolZ*0.5 + fzk cos, used iff inside [0,VolX-2]x[0,VolZ-2].
+// Tables: per ray its sample interval, per 64-ray tile the union and the
+// common core of those intervals, per (tilt, k) the row offset, the z weight
+// and fzk*sin.
+bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
+                     Geometry& g)
+{
+    const int end_x = vol_x - 2, end_z = vol_z - 2;
+    const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
+    g.fp_len.assign(n, 0);
+    g.fp_geo_off.assign(n, 0);
+    size_t total = 0;
+    for (int t = 0; t < n; t++) {
+        double c = cs[2 * t];
+        if (!(c > 0.0)) return false;
+        double l = vol_z / c + 1.5;
+        if (!(l < 1e6)) return false;
+        int len = (int)l;
+        g.fp_len[t] = len;
+        g.fp_geo_off[t] = (int)total;
+        total += (size_t)len;
+    }
+    g.fp_geo.assign(total, make_float4(0, 0, 0, 0));
+    g.fp_range.assign((size_t)n * proj, make_int2(0, -1));
+    g.fp_tile.assign((size_t)n * ntile, make_int4(0, 0, -1, -1));
+    parallel_tilts(n, [&](int t) {
+        double c = cs[2 * t], s = cs[2 * t + 1];
+        int len = g.fp_len[t];
+        float4* geo = g.fp_geo.data() + g.fp_geo_off[t];
+        double tz = vol_z * 0.5;
+        int kz_lo = len, kz_hi = -1;
+        for (int k = 0; k < len; k++) {
+            double fzk = k - len * 0.5;
+            double z = tz + fzk * c;
+            bool zin = !(z < 0 || z > end_z);
+            if (zin) {
+                kz_lo = std::min(kz_lo, k);
+                kz_hi = std::max(kz_hi, k);
+            }
+            double zc = std::min(std::max(z, 0.0), (double)end_z);
+            int iz = (int)zc;
+            float wz = (float)(zc - iz);
+            float q = (float)(fzk * s);
+            geo[k] = make_float4(__builtin_bit_cast(float, iz * vol_x), wz, q, 0.0f);
+        }
+        for (int xd = 0; xd < proj; xd++) {
+            double xp = xd - 0.5 * proj;
+            double tx = xp / c + vol_x * 0.5;
+            int lo = -1, hi = -2;
+            for (int k = std::max(kz_lo, 0); k <= kz_hi; k++) {
+                double fzk = k - len * 0.5;
+                double x = tx - fzk * s;
+                double z = tz + fzk * c;
+                bool in = !(x < 0 || z < 0 || x > end_x || z > end_z);
+                if (in) {
+                    if (lo < 0) lo = k;
+                    hi = k;
+                } else if (lo >= 0) {
+                    break;
+                }
+            }
+            g.fp_range[(size_t)t * proj + xd] = lo < 0 ? make_int2(0, -1) : make_int2(lo, hi);
+        }
+        for (int w = 0; w < ntile; w++) {
+            int out_lo = 1 << 30, out_hi = -1, core_lo = -1, core_hi = 1 << 30;
+            bool all = true;
+            for (int xd = w * FP_WAVE; xd < std::min((w + 1) * FP_WAVE, proj); xd++) {
+                int2 r = g.fp_range[(size_t)t * proj + xd];
+                if (r.y < r.x) {
+                    all = false;
+                    continue;
+                }
+                out_lo = std::min(out_lo, r.x);
+                out_hi = std::max(out_hi, r.y);
+                core_lo = std::max(core_lo, r.x);
+                core_hi = std::min(core_hi, r.y);
+            }
+            int4 tl;
+            if (out_hi < 0) tl = make_int4(0, 0, -1, -1);
+            else if (!all || core_lo > core_hi) tl = make_int4(out_lo, out_hi + 1, out_hi, out_hi);
+            else tl = make_int4(out_lo, core_lo, core_hi, out_hi);
+            g.fp_tile[(size_t)t * ntile + w] = tl;
+        }
+    });
+    return true;
+}
+
+// Factorisation into the radices fft_lds.h provides.
+bool make_fft(int n, FftDesc& fd)
+{
+    fd = FftDesc{};
+    fd.n = n;
+    int m = n, s = 0;
+    if (n < 2) return false;
+    while (m % 4 == 0 && s < ATR_FFT_MAX_STAGES) { fd.radix[s++] = 4; m /= 4; }
+    while (m % 2 == 0 && s < ATR_FFT_MAX_STAGES) { fd.radix[s++] = 2; m /= 2; }
+    while (m % 3 == 0 && s < ATR_FFT_MAX_STAGES) { fd.radix[s++] = 3; m /= 3; }
+    while (m % 5 == 0 && s < ATR_FFT_MAX_STAGES) { fd.radix[s++] = 5; m /= 5; }
+    fd.n_stages = s;
+    return m == 1;
+}
+
+std::vector<float2> twiddles(int n)
+{
+    std::vector<float2> w(n);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int e = 0; e < n; e++) {
+        double a = two_pi * (double)e / (double)n;
+        w[e] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    return w;
+}
+
+// Recon/GRWeight.cu:7-23: bin x of the length-n row spectrum is scaled by
+// 2 r (0.55 + 0.45 cos(6.2831852 r)) / n, r = x/n (evaluated in double).
+std::vector<float> ramp_filter(int n)
+{
+    std::vector<float> w(n / 2 + 1);
+    const double c2pi = (double)6.2831852f;
+    for (int x = 0; x <= n / 2; x++) {
+        double r = (double)x / (double)n;
+        w[x] = (float)(2.0 * r * (0.55 + 0.45 * std::cos(c2pi * r)) / (double)n);
+    }
+    return w;
+}
+
+}  // namespace atr

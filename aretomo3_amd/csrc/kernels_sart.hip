@@ -1,0 +1,571 @@
+// kernels_sart.hip -- pre-stage, weighting, SART back-projection, forward
+// projection + difference, and the volume fill/output kernels (gfx950).
+//
+// Data layout in HBM (one series):
+//   d_aligned [N][aln_y][aln_x]      pre-stage output, -1e30 where empty
+//   d_sino    [VolY][N][stride]      weighted projections, sinogram-major:
+//                                    the sinogram of slice y is contiguous
+//   d_diff    [VolY][max_subset][stride]  SART residual of the current subset
+//   d_vol     [VolY][VolZ][VolX]     SART state, x fastest (XZY, unflipped)
+#include <climits>
+
+#include "atr_internal.h"
+#include "storage.h"
+
+namespace atr {
+
+// ------------------------------------------------------------------------
+// Pre-stage: Correct/GCorrPatchShift.cu:58-100 (global shift + tilt-axis
+// rotation, nearest neighbour, -1e30 outside the raw image), geometry in
+// FP64 with contraction off so that the rounding of every coordinate is the
+// rounding of the FP64 restatement.  One workgroup per (aligned row, tilt);
+// the row's (sum, count) of non-empty pixels is written for the fill mean.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_prestage(const float* __restrict__ in, size_t tilt_stride,
+                                                  size_t row_stride, int in_x, int in_y,
+                                                  const double2* __restrict__ axis_cs,
+                                                  const float2* __restrict__ shifts,
+                                                  float* __restrict__ out, int out_x, int out_y,
+                                                  double2* __restrict__ row_stats)
+{
+#pragma clang fp contract(off)
+    const int y = blockIdx.x, t = blockIdx.y;
+    const double ca = axis_cs[t].x, sa = axis_cs[t].y;
+    const double sx = (double)shifts[t].x + in_x * 0.5;
+    const double sy = (double)shifts[t].y + in_y * 0.5;
+    const double yc = y + 0.5 - out_y * 0.5;
+    const float* src = in + (size_t)t * tilt_stride;
+    float* dst = out + ((size_t)t * out_y + y) * out_x;
+    double sum = 0.0;
+    int cnt = 0;
+    for (int x = threadIdx.x; x < out_x; x += blockDim.x) {
+        double xc = x + 0.5 - out_x * 0.5;
+        double xr = xc * ca - yc * sa;
+        double yr = xc * sa + yc * ca;
+        xr += sx;
+        yr += sy;
+        int ix = (int)(xr + 0.5);
+        int iy = (int)(yr + 0.5);
+        float v = kSentinel;
+        if (ix >= 0 && ix < in_x && iy >= 0 && iy < in_y) {
+            v = src[(size_t)iy * row_stride + ix];
+            if (!(v < kEmpty)) {
+                sum += v;
+                cnt += 1;
+            }
+        }
+        dst[x] = v;
+    }
+    // fixed-order block reduction (deterministic)
+    __shared__ double s_sum[256];
+    __shared__ int s_cnt[256];
+    s_sum[threadIdx.x] = sum;
+    s_cnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) row_stats[(size_t)t * out_y + y] = make_double2(s_sum[0], (double)s_cnt[0]);
+}
+
+// Per-tilt fill statistics: sum of the row partials in a fixed order.
+__global__ __launch_bounds__(256) void k_tilt_stats(const double2* __restrict__ row_stats, int rows,
+                                                    double2* __restrict__ tilt_stats)
+{
+    const int t = blockIdx.x;
+    double sum = 0.0, cnt = 0.0;
+    for (int y = threadIdx.x; y < rows; y += blockDim.x) {
+        double2 r = row_stats[(size_t)t * rows + y];
+        sum += r.x;
+        cnt += r.y;
+    }
+    __shared__ double s_sum[256], s_cnt[256];
+    s_sum[threadIdx.x] = sum;
+    s_cnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tilt_stats[t] = make_double2(s_sum[0], s_cnt[0]);
+}
+
+// Value of aligned pixel i of tilt t after MaUtil/GFillEmpty2D.cu:60-88:
+// an empty pixel follows the random chain (<= 100 hops) to the first
+// non-empty pixel, else takes the tilt mean.  Reads only the unfilled
+// image, so every pixel can be resolved independently.
+__device__ __forceinline__ float filled_value(const float* __restrict__ img, int i,
+                                              const int* __restrict__ table, float mean)
+{
+    float v = img[i];
+    if (!(v < kEmpty)) return v;
+    int next = table[i];
+    for (int j = 0; j < 100; j++) {
+        float w = img[next];
+        if (w < kEmpty) {
+            next = table[next];
+            continue;
+        }
+        return w;
+    }
+    return mean;
+}
+
+// ------------------------------------------------------------------------
+// Weighting (Recon/GWeightProjs.cu:7-19): s *= cos_t / VolZ, written
+// sinogram-major into d_sino, with the pre-stage random fill resolved on the
+// fly.  One thread per pixel; grid (x-blocks, tilt, y).
+// ------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(256) void k_weight(const float* __restrict__ in, size_t tilt_stride,
+                                                size_t row_stride, int fill,
+                                                const int* __restrict__ table,
+                                                const double2* __restrict__ tilt_stats,
+                                                const float2* __restrict__ cs, int n, int proj,
+                                                int vol_z, S* __restrict__ sino, int stride,
+                                                int* __restrict__ sflag)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y, y = blockIdx.z;
+    if (x >= stride) return;
+    S* dst = sino + ((size_t)y * n + t) * stride;
+    if (x >= proj) {
+        st<S>(dst + x, 0.0f);
+        return;
+    }
+    const float w = cs[t].x / (float)vol_z;
+    const float* img = in + (size_t)t * tilt_stride;
+    float v;
+    if (fill) {
+        double2 ts = tilt_stats[t];
+        float mean = ts.y > 0 ? (float)(ts.x / ts.y) : 0.0f;
+        v = filled_value(img, (int)(y * row_stride + x), table, mean);
+    } else {
+        v = img[(size_t)y * row_stride + x];
+    }
+    v *= w;
+    if (v <= kEmpty) atomicOr(sflag + y, 1);
+    st<S>(dst + x, v);
+}
+
+// ------------------------------------------------------------------------
+// SART back-projection (Recon/GBackProj.cu:12-60).
+//
+// Workgroup = BP_TX x BP_TZ voxels of one slice, lane = x, each thread walks
+// BP_TZ z-rows.  For each chunk of BP_CHUNK tilts the needed part of every
+// sinogram row is staged in LDS as pairs (p[i], p[i+1]-p[i]) so that one
+// ds_read_b64 and one FMA give the linear interpolation.  Inclusion comes
+// from the FP64 interval tables: tiles the host classified as fully inside
+// skip all per-lane tests and count per tile; partial tiles test x against
+// the (tilt, z) interval.  Sums and counts are FP32/int per voxel, in tilt
+// order, as the reference.
+// ------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(BP_TX) void k_back_proj(
+    const S* __restrict__ sino, int rows_per_slice, int stride, int proj,
+    const TiltRow* __restrict__ list, int n_list, const float2* __restrict__ cs,
+    const int2* __restrict__ lohi, const uint8_t* __restrict__ cls, int n_ztiles, int n_xtiles,
+    float relax, int positivity, const int* __restrict__ sflag, float* __restrict__ vol,
+    int vol_x, int vol_z, int* __restrict__ vflag)
+{
+    __shared__ float2 pairs[BP_CHUNK][BP_WIN];
+    __shared__ int win0[BP_CHUNK];
+    const int tid = threadIdx.x;
+    const int xt = blockIdx.x, zt = blockIdx.y, y = blockIdx.z;
+    const int x0 = xt * BP_TX, z0 = zt * BP_TZ;
+    const int x = x0 + tid;
+    const int xc = min(x, vol_x - 1);
+    const bool sent = sflag != nullptr && sflag[y] != 0;
+    const float thr = empty_threshold<S>();
+
+    float acc[BP_TZ];
+    int cnt[BP_TZ];
+#pragma unroll
+    for (int k = 0; k < BP_TZ; k++) {
+        acc[k] = 0.0f;
+        cnt[k] = 0;
+    }
+    int full_cnt = 0;
+    const double fx = (double)xc - vol_x * 0.5;
+    const double fz0 = (double)z0 - vol_z * 0.5;
+    const float fxt0 = (float)((double)x0 - vol_x * 0.5);
+    const float fxt1 = (float)((double)(x0 + BP_TX - 1) - vol_x * 0.5);
+    const float fzt1 = (float)(fz0 + (BP_TZ - 1));
+    const S* base = sino + (size_t)y * rows_per_slice * stride;
+
+    for (int c0 = 0; c0 < n_list; c0 += BP_CHUNK) {
+        const int nc = min(BP_CHUNK, n_list - c0);
+        // ---- stage the row segments of this chunk ----
+        for (int j = 0; j < nc; j++) {
+            const TiltRow tr = list[c0 + j];
+            const uint8_t cl = cls[((size_t)tr.tilt * n_ztiles + zt) * n_xtiles + xt];
+            if (cl == 0) {
+                if (tid == 0) win0[j] = INT_MIN;
+                continue;
+            }
+            const float c = cs[tr.tilt].x, s = cs[tr.tilt].y;
+            float umin = 0.5f * proj + fminf(c * fxt0, c * fxt1) + fminf(s * (float)fz0, s * fzt1);
+            int w0 = (int)floorf(umin) - 3;
+            const S* row = base + (size_t)tr.row * stride;
+            for (int i = tid; i < BP_WIN; i += BP_TX) {
+                int g = w0 + i;
+                float2 pr = make_float2(0.0f, 0.0f);
+                if (g >= -1 && g <= proj - 1) {
+                    int ga = max(g, 0);
+                    int gb = min(ga + 1, proj - 1);
+                    float p = ld<S>(row + ga), q = ld<S>(row + gb);
+                    if (p <= thr) p = kSentinel;
+                    if (q <= thr) q = kSentinel;
+                    if (g == -1) pr = make_float2(p - (q - p), q - p);
+                    else if (g == proj - 1) pr = make_float2(p, 0.0f);
+                    else pr = make_float2(p, q - p);
+                }
+                pairs[j][i] = pr;
+            }
+            if (tid == 0) win0[j] = w0;
+        }
+        __syncthreads();
+        // ---- accumulate ----
+        for (int j = 0; j < nc; j++) {
+            const int w0 = win0[j];
+            if (w0 == INT_MIN) continue;
+            const TiltRow tr = list[c0 + j];
+            const uint8_t cl = cls[((size_t)tr.tilt * n_ztiles + zt) * n_xtiles + xt];
+            const float c = cs[tr.tilt].x, s = cs[tr.tilt].y;
+            const double kk = fz0 * (double)s + proj / 2.0 - (double)w0;
+            const float u0 = (float)(fx * (double)c + kk);
+            const float2* pj = pairs[j];
+            if (cl == 1 && !sent) {
+                full_cnt += 1;
+#pragma unroll
+                for (int k = 0; k < BP_TZ; k++) {
+                    float u = fmaf((float)k, s, u0);
+                    int i = (int)u;
+                    float f = __builtin_amdgcn_fractf(u);
+                    float2 pr = pj[i];
+                    acc[k] += fmaf(f, pr.y, pr.x);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < BP_TZ; k++) {
+                    const int z = min(z0 + k, vol_z - 1);
+                    const int2 r = lohi[(size_t)tr.tilt * vol_z + z];
+                    float u = fmaf((float)k, s, u0);
+                    int i = (int)u;
+                    float f = __builtin_amdgcn_fractf(u);
+                    float2 pr = pj[i];
+                    float v = fmaf(f, pr.y, pr.x);
+                    bool in = (x >= r.x) && (x <= r.y) && !(v <= kEmpty);
+                    acc[k] += in ? v : 0.0f;
+                    cnt[k] += in ? 1 : 0;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (x >= vol_x) return;
+    bool wrote_sent = false;
+#pragma unroll
+    for (int k = 0; k < BP_TZ; k++) {
+        const int z = z0 + k;
+        if (z >= vol_z) break;
+        const int count = full_cnt + cnt[k];
+        if (count <= 0) continue;
+        const size_t vi = ((size_t)y * vol_z + z) * vol_x + x;
+        float v = relax * acc[k] / (float)count + vol[vi];
+        if (v > kEmpty) {
+            vol[vi] = positivity ? fmaxf(v, 0.0f) : v;
+        } else {
+            vol[vi] = kSentinel;
+            wrote_sent = true;
+        }
+    }
+    if (wrote_sent) atomicOr(vflag + y, 1);
+}
+
+// ------------------------------------------------------------------------
+// Forward projection + difference (Recon/GForProj.cu:10-76 and
+// Recon/GDiffProj.cu:7-21 fused).  One ray per lane, 256 consecutive
+// detector pixels of one tilt and slice per workgroup.  The FP64 tables give
+// each ray's sample interval, each 64-ray tile's union/core of intervals, and
+// per sample k the row offset, z weight and fzk*sin, so the inner loop is
+// FP32 bilinear interpolation with no bounds tests in the core.  The
+// difference raw - fwd (0 where either is empty) is written for the BP.
+// ------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(FP_RAYS) void k_for_proj_diff(
+    const float* __restrict__ vol, int vol_x, int vol_z, const int* __restrict__ vflag,
+    const int* __restrict__ fp_list, const float2* __restrict__ cs,
+    const int2* __restrict__ fp_range, const int4* __restrict__ fp_tile,
+    const int* __restrict__ geo_off, const float4* __restrict__ geo, int proj,
+    const S* __restrict__ sino, int n, int stride, S* __restrict__ diff, int diff_rows)
+{
+    const int j = blockIdx.y, y = blockIdx.z;
+    const int t = fp_list[j];
+    const int xd = blockIdx.x * FP_RAYS + threadIdx.x;
+    const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
+    const int wtile = min((blockIdx.x * FP_RAYS + (int)(threadIdx.x & ~(FP_WAVE - 1))) / FP_WAVE,
+                          ntile - 1);
+    const bool lane_ok = xd < proj;
+    const float c = cs[t].x;
+    const float tf = (float)(((double)xd - 0.5 * proj) / (double)c + vol_x * 0.5);
+    const int2 rr = lane_ok ? fp_range[(size_t)t * proj + xd] : make_int2(0, -1);
+    const int4 tl = fp_tile[(size_t)t * ntile + wtile];
+    const float* V = vol + (size_t)y * vol_z * vol_x;
+    const float4* G = geo + geo_off[t];
+    const bool sent = vflag[y] != 0;
+    const float xmax = (float)(vol_x - 2);
+
+    float sum = 0.0f;
+    int cnt = 0;
+    auto sample = [&](int k) -> float {
+        const float4 g = G[k];
+        float xs = __builtin_amdgcn_fmed3f(tf - g.z, 0.0f, xmax);
+        int ix = (int)xs;
+        float fxs = __builtin_amdgcn_fractf(xs);
+        const float* r0 = V + __float_as_int(g.x) + ix;
+        const float* r1 = r0 + vol_x;
+        float a = r0[0], b = r0[1], cc = r1[0], d = r1[1];
+        float top = fmaf(fxs, b - a, a);
+        float bot = fmaf(fxs, d - cc, cc);
+        return fmaf(g.y, bot - top, top);
+    };
+    if (!sent) {
+        for (int k = tl.x; k < tl.y; k++) {
+            float v = sample(k);
+            sum += (k >= rr.x && k <= rr.y) ? v : 0.0f;
+        }
+        for (int k = tl.y; k <= tl.z; k++) sum += sample(k);
+        for (int k = tl.z + 1; k <= tl.w; k++) {
+            float v = sample(k);
+            sum += (k >= rr.x && k <= rr.y) ? v : 0.0f;
+        }
+        cnt = max(rr.y - rr.x + 1, 0);
+    } else {
+        for (int k = tl.x; k <= tl.w; k++) {
+            float v = sample(k);
+            bool in = k >= rr.x && k <= rr.y && !(v < kEmpty);
+            sum += in ? v : 0.0f;
+            cnt += in ? 1 : 0;
+        }
+    }
+    if (!lane_ok) return;
+    const float fwd = cnt <= 0 ? kSentinel : sum / (float)cnt;
+    float raw = ld<S>(sino + ((size_t)y * n + t) * stride + xd);
+    if (raw <= empty_threshold<S>()) raw = kSentinel;
+    float d = (fwd < kEmpty || raw < kEmpty) ? 0.0f : raw - fwd;
+    st<S>(diff + ((size_t)y * diff_rows + j) * stride + xd, d);
+}
+
+// ------------------------------------------------------------------------
+// Volume helpers: zero / load the SART state, per-slice fill statistics,
+// output with the fill of MaUtil/GFillEmpty2D.cu:60-88 (CTomoSart.cpp:86-89)
+// and the z-flip / XYZ layouts of CDoSartRecon.cpp:253-269 and
+// DataUtil/CTiltSeries.cpp:286-305.
+// ------------------------------------------------------------------------
+__global__ void k_fill_zero(float* __restrict__ p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+
+__global__ void k_fill_zero_int(int* __restrict__ p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// SLICE-layout caller volume -> state, flagging slices that hold empties.
+__global__ __launch_bounds__(256) void k_load_volume(const float* __restrict__ in,
+                                                     float* __restrict__ vol, int vol_x,
+                                                     int vol_z, int* __restrict__ vflag)
+{
+    const int y = blockIdx.y;
+    const size_t n = (size_t)vol_x * vol_z;
+    bool any = false;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        float v = in[(size_t)y * n + i];
+        vol[(size_t)y * n + i] = v;
+        any |= v < kEmpty;
+    }
+    if (any) atomicOr(vflag + y, 1);
+}
+
+// (sum, count) of non-empty voxels of each flagged slice, fixed order.
+__global__ __launch_bounds__(256) void k_vol_stats(const float* __restrict__ vol, int vol_x,
+                                                   int vol_z, const int* __restrict__ vflag,
+                                                   double2* __restrict__ stats)
+{
+    const int y = blockIdx.x;
+    if (vflag[y] == 0) return;
+    const size_t n = (size_t)vol_x * vol_z;
+    const float* v = vol + (size_t)y * n;
+    double sum = 0.0, cnt = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
+        float a = v[i];
+        if (!(a < kEmpty)) {
+            sum += a;
+            cnt += 1.0;
+        }
+    }
+    __shared__ double s_sum[256], s_cnt[256];
+    s_sum[threadIdx.x] = sum;
+    s_cnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) stats[y] = make_double2(s_sum[0], s_cnt[0]);
+}
+
+__device__ __forceinline__ size_t out_index(int layout, int x, int y, int z, int vol_x,
+                                            int vol_y, int vol_z)
+{
+    switch (layout) {
+    case ATR_LAYOUT_XYZ: return ((size_t)z * vol_y + y) * vol_x + x;
+    case ATR_LAYOUT_XYZ_NOFLIP: return ((size_t)(vol_z - 1 - z) * vol_y + y) * vol_x + x;
+    case ATR_LAYOUT_SLICE: return ((size_t)y * vol_z + z) * vol_x + x;
+    default: return ((size_t)y * vol_z + (vol_z - 1 - z)) * vol_x + x;
+    }
+}
+
+// grid (x-blocks, z, y)
+__global__ __launch_bounds__(256) void k_output(const float* __restrict__ vol, int vol_x,
+                                                int vol_y, int vol_z, const int* __restrict__ vflag,
+                                                const double2* __restrict__ stats,
+                                                const int* __restrict__ table, int layout,
+                                                float* __restrict__ out)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y, y = blockIdx.z;
+    if (x >= vol_x) return;
+    const float* slice = vol + (size_t)y * vol_z * vol_x;
+    float v = slice[(size_t)z * vol_x + x];
+    if (vflag[y] != 0 && v < kEmpty) {
+        double2 st2 = stats[y];
+        float mean = st2.y > 0 ? (float)(st2.x / st2.y) : 0.0f;
+        v = filled_value(slice, z * vol_x + x, table, mean);
+    }
+    out[out_index(layout, x, y, z, vol_x, vol_y, vol_z)] = v;
+}
+
+// ------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------
+void launch_prestage(const Plan& p, const float* in, hipStream_t s)
+{
+    dim3 grid(p.aln_y, p.n);
+    hipLaunchKernelGGL(k_prestage, grid, dim3(256), 0, s, in, p.in_tilt_stride, p.in_row_stride,
+                       p.in_x, p.in_y, (const double2*)p.d_axis_cs, (const float2*)p.d_shifts,
+                       p.d_aligned, p.aln_x, p.aln_y, (double2*)p.d_row_stats);
+}
+
+void launch_tilt_stats(const Plan& p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tilt_stats, dim3(p.n), dim3(256), 0, s, (const double2*)p.d_row_stats,
+                       p.aln_y, (double2*)p.d_tilt_stats);
+}
+
+template <typename S>
+static void launch_weight_t(const Plan& p, const float* in, hipStream_t s)
+{
+    const float* src = p.prestage ? p.d_aligned : in;
+    size_t ts = p.prestage ? (size_t)p.aln_x * p.aln_y : p.in_tilt_stride;
+    size_t rs = p.prestage ? (size_t)p.aln_x : p.in_row_stride;
+    dim3 grid((p.sino_stride + 255) / 256, p.n, p.vol_y);
+    hipLaunchKernelGGL(k_weight<S>, grid, dim3(256), 0, s, src, ts, rs, p.fill ? 1 : 0,
+                       p.d_fill_aln, (const double2*)p.d_tilt_stats, p.d_cs, p.n, p.proj,
+                       p.vol_z, (S*)p.d_sino, p.sino_stride, p.d_sflag);
+}
+
+void launch_weight_sart(const Plan& p, const float* in, hipStream_t s)
+{
+    if (p.storage == ATR_STORAGE_F16) launch_weight_t<_Float16>(p, in, s);
+    else if (p.storage == ATR_STORAGE_BF16) launch_weight_t<bf16_t>(p, in, s);
+    else launch_weight_t<float>(p, in, s);
+}
+
+template <typename S>
+static void launch_bp_t(const Plan& p, const void* sino, int rows, const TiltRow* list, int n_list,
+                        float relax, bool check_sent, hipStream_t s)
+{
+    int nxt = (p.vol_x + BP_TX - 1) / BP_TX, nzt = (p.vol_z + BP_TZ - 1) / BP_TZ;
+    dim3 grid(nxt, nzt, p.vol_y);
+    hipLaunchKernelGGL(k_back_proj<S>, grid, dim3(BP_TX), 0, s, (const S*)sino, rows, p.sino_stride,
+                       p.proj, list, n_list, p.d_cs, p.d_bp_lohi, p.d_bp_class, nzt, nxt, relax,
+                       p.positivity, check_sent ? p.d_sflag : nullptr, p.d_vol, p.vol_x, p.vol_z,
+                       p.d_vflag);
+}
+
+void launch_bp(const Plan& p, const void* sino, int rows, const TiltRow* list, int n_list,
+               float relax, bool check_sent, hipStream_t s)
+{
+    if (p.storage == ATR_STORAGE_F16) launch_bp_t<_Float16>(p, sino, rows, list, n_list, relax, check_sent, s);
+    else if (p.storage == ATR_STORAGE_BF16) launch_bp_t<bf16_t>(p, sino, rows, list, n_list, relax, check_sent, s);
+    else launch_bp_t<float>(p, sino, rows, list, n_list, relax, check_sent, s);
+}
+
+template <typename S>
+static void launch_fp_t(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
+{
+    dim3 grid((p.proj + FP_RAYS - 1) / FP_RAYS, n_list, p.vol_y);
+    hipLaunchKernelGGL(k_for_proj_diff<S>, grid, dim3(FP_RAYS), 0, s, p.d_vol, p.vol_x, p.vol_z,
+                       p.d_vflag, fp_list, p.d_cs, p.d_fp_range, p.d_fp_tile, p.d_fp_geo_off,
+                       p.d_fp_geo, p.proj, (const S*)p.d_sino, p.n, p.sino_stride, (S*)p.d_diff,
+                       p.max_subset);
+}
+
+void launch_fp_diff(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
+{
+    if (p.storage == ATR_STORAGE_F16) launch_fp_t<_Float16>(p, fp_list, n_list, s);
+    else if (p.storage == ATR_STORAGE_BF16) launch_fp_t<bf16_t>(p, fp_list, n_list, s);
+    else launch_fp_t<float>(p, fp_list, n_list, s);
+}
+
+void launch_zero(float* ptr, size_t n, hipStream_t s)
+{
+    size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, s, ptr, n);
+}
+
+void launch_zero_int(int* ptr, int n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_zero_int, dim3((n + 255) / 256), dim3(256), 0, s, ptr, n);
+}
+
+void launch_load_volume(const Plan& p, const float* in, hipStream_t s)
+{
+    size_t n = (size_t)p.vol_x * p.vol_z;
+    unsigned bx = (unsigned)std::min<size_t>((n + 255) / 256, 64);
+    hipLaunchKernelGGL(k_load_volume, dim3(bx, p.vol_y), dim3(256), 0, s, in, p.d_vol, p.vol_x,
+                       p.vol_z, p.d_vflag);
+}
+
+void launch_vol_stats(const Plan& p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_vol_stats, dim3(p.vol_y), dim3(256), 0, s, p.d_vol, p.vol_x, p.vol_z,
+                       p.d_vflag, (double2*)p.d_vstats);
+}
+
+void launch_output(const Plan& p, float* out, hipStream_t s)
+{
+    dim3 grid((p.vol_x + 255) / 256, p.vol_z, p.vol_y);
+    hipLaunchKernelGGL(k_output, grid, dim3(256), 0, s, p.d_vol, p.vol_x, p.vol_y, p.vol_z,
+                       p.d_vflag, (const double2*)p.d_vstats, p.d_fill_vol, p.layout, out);
+}
+
+}  // namespace atr

@@ -1,0 +1,330 @@
+// kernels_wbp.hip -- weighted back-projection path (gfx950):
+//   k_filter        weighting + ramp/Hamming filter, LDS row FFT
+//                   (Recon/GWeightProjs.cu:7-19, Recon/GRWeight.cu:7-69)
+//   k_wbp_bp_rowfft 2x-grid back-projection of two grid rows + their row
+//                   FFT, keeping the bins the Fourier crop uses
+//                   (Recon/GBackProjWbp.cu:14-53,126)
+//   k_wbp_colfft    column FFT, normalisation, crop, inverse column FFT
+//                   (MaUtil/CCufft2D.cpp:65-80, MaUtil/GFourierResize2D.cu:7-39)
+//   k_wbp_rowc2r    real inverse row FFT + output layout
+//                   (MaUtil/CCufft2D.cpp:108-123, MaUtil/CPad2D.cpp:29-39,
+//                    Recon/CDoWbpRecon.cpp:107-123)
+// Scratch: d_wbp_t [slab][2VolZ][VolX/2+1], d_wbp_u [slab][VolZ][VolX/2+1].
+#include "atr_internal.h"
+#include "storage.h"
+
+namespace atr {
+
+__device__ __forceinline__ float filled_value_w(const float* __restrict__ img, int i,
+                                                const int* __restrict__ table, float mean)
+{
+    float v = img[i];
+    if (!(v < kEmpty)) return v;
+    int next = table[i];
+    for (int j = 0; j < 100; j++) {
+        float w = img[next];
+        if (w < kEmpty) {
+            next = table[next];
+            continue;
+        }
+        return w;
+    }
+    return mean;
+}
+
+// One workgroup per (tilt, pair of rows): the two weighted rows are packed as
+// real and imaginary parts of one complex row, transformed, scaled by the
+// real even filter (which keeps the two parts separate), transformed back.
+template <typename S>
+__global__ __launch_bounds__(256) void k_filter(const float* __restrict__ in, size_t tilt_stride,
+                                                size_t row_stride, int fill,
+                                                const int* __restrict__ table,
+                                                const double2* __restrict__ tilt_stats,
+                                                const float2* __restrict__ cs, int n, int rows,
+                                                int proj, int vol_z, FftDesc fd,
+                                                const float* __restrict__ filt,
+                                                S* __restrict__ sino, int stride,
+                                                int* __restrict__ sflag)
+{
+    extern __shared__ float2 lds[];
+    float2* a = lds;
+    float2* b = lds + proj;
+    const int t = blockIdx.y;
+    const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
+    const bool has1 = y1 < rows;
+    const float w = cs[t].x / (float)vol_z;
+    const float* img = in + (size_t)t * tilt_stride;
+    float mean = 0.0f;
+    if (fill) {
+        double2 ts = tilt_stats[t];
+        mean = ts.y > 0 ? (float)(ts.x / ts.y) : 0.0f;
+    }
+    for (int x = threadIdx.x; x < proj; x += blockDim.x) {
+        float va, vb = 0.0f;
+        if (fill) {
+            va = filled_value_w(img, (int)(y0 * row_stride + x), table, mean);
+            if (has1) vb = filled_value_w(img, (int)(y1 * row_stride + x), table, mean);
+        } else {
+            va = img[(size_t)y0 * row_stride + x];
+            if (has1) vb = img[(size_t)y1 * row_stride + x];
+        }
+        a[x] = make_float2(va * w, vb * w);
+    }
+    __syncthreads();
+    float2* r = lds_fft(a, b, fd, 1, -1.0f, threadIdx.x, blockDim.x);
+    for (int k = threadIdx.x; k < proj; k += blockDim.x) {
+        int kk = k <= proj / 2 ? k : proj - k;
+        float f = filt[kk];
+        r[k] = make_float2(r[k].x * f, r[k].y * f);
+    }
+    __syncthreads();
+    float2* o = (r == a) ? b : a;
+    r = lds_fft(r, o, fd, 1, 1.0f, threadIdx.x, blockDim.x);
+    S* d0 = sino + ((size_t)y0 * n + t) * stride;
+    S* d1 = sino + ((size_t)y1 * n + t) * stride;
+    bool sent = false;
+    for (int x = threadIdx.x; x < stride; x += blockDim.x) {
+        float va = x < proj ? r[x].x : 0.0f;
+        float vb = x < proj ? r[x].y : 0.0f;
+        sent |= va <= kEmpty;
+        st<S>(d0 + x, va);
+        if (has1) {
+            sent |= vb <= kEmpty;
+            st<S>(d1 + x, vb);
+        }
+    }
+    if (sent) {
+        atomicOr(sflag + y0, 1);
+        if (has1) atomicOr(sflag + y1, 1);
+    }
+}
+
+// One workgroup per (slice, pair of 2x-grid rows).  Every grid point is the
+// mean of the linear interpolations of the tilts whose FP64 inclusion
+// interval holds it (0 when none).  The two rows are transformed together as
+// one complex row of length 2VolX; bins 0..VolX/2 of each are written.
+template <typename S>
+__global__ __launch_bounds__(256) void k_wbp_bp_rowfft(
+    const S* __restrict__ sino, int n, int stride, int proj, const float2* __restrict__ cs,
+    const unsigned char* __restrict__ excl, const int2* __restrict__ lohi,
+    const int* __restrict__ sflag, int vol_x, int vol_z, int y0, FftDesc fd,
+    float2* __restrict__ tbuf)
+{
+    extern __shared__ float2 lds[];
+    const int nx2 = 2 * vol_x, nz2 = 2 * vol_z, kxn = vol_x / 2 + 1;
+    float2* a = lds;
+    float2* b = lds + nx2;
+    const int zp = blockIdx.x, ys = blockIdx.y, y = y0 + ys;
+    const int za = 2 * zp, zb = za + 1;
+    const S* base = sino + (size_t)y * n * stride;
+    const bool sent = sflag[y] != 0;
+    const float thr = empty_threshold<S>();
+    const int cx = proj / 2;
+    for (int x = threadIdx.x; x < nx2; x += blockDim.x) {
+        const float fx = (float)((double)x - nx2 / 2.0);
+        float sa = 0.0f, sb = 0.0f;
+        int ca = 0, cb = 0;
+        for (int t = 0; t < n; t++) {
+            if (excl[t]) continue;
+            const float c = cs[t].x, s = cs[t].y;
+            const float hc = 0.5f * c;
+            const S* row = base + (size_t)t * stride;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int z = h == 0 ? za : zb;
+                const int2 r = lohi[(size_t)t * nz2 + z];
+                if (x < r.x || x > r.y) continue;
+                const double fz = (double)z - nz2 / 2.0;
+                const float kk = (float)(fz * (double)s / 2 + cx + 0.5);
+                float u = fmaf(fx, hc, kk);
+                u = __builtin_amdgcn_fmed3f(u, 0.0f, (float)(proj - 2));
+                int iu = (int)u;
+                float f = __builtin_amdgcn_fractf(u);
+                float p0 = ld<S>(row + iu), p1 = ld<S>(row + iu + 1);
+                if (sent) {
+                    if (p0 <= thr) p0 = kSentinel;
+                    if (p1 <= thr) p1 = kSentinel;
+                }
+                float v = p0 * (1.0f - f) + p1 * f;
+                if (sent && v <= kEmpty) continue;
+                if (h == 0) {
+                    sa += v;
+                    ca++;
+                } else {
+                    sb += v;
+                    cb++;
+                }
+            }
+        }
+        float va = ca > 0 ? sa / (float)ca : 0.0f;
+        float vb = cb > 0 ? sb / (float)cb : 0.0f;
+        a[x] = make_float2(va, vb);
+    }
+    __syncthreads();
+    float2* r = lds_fft(a, b, fd, 1, -1.0f, threadIdx.x, blockDim.x);
+    float2* ta = tbuf + ((size_t)ys * nz2 + za) * kxn;
+    float2* tb = tbuf + ((size_t)ys * nz2 + zb) * kxn;
+    for (int k = threadIdx.x; k < kxn; k += blockDim.x) {
+        float2 z1 = r[k];
+        float2 z2 = r[(nx2 - k) % nx2];
+        // A = (Z[k] + conj(Z[N-k]))/2, B = (Z[k] - conj(Z[N-k]))/(2i)
+        ta[k] = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+        tb[k] = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
+    }
+}
+
+// One workgroup per (slice, block of kb spectrum columns): forward FFT along
+// z (length 2VolZ), scale by 1/(2VolX*2VolZ) (MaUtil/CCufft2D.cpp:75-78),
+// crop to VolZ rows (row j <= VolZ/2 from j, else from j+VolZ), inverse FFT
+// along z (length VolZ).
+__global__ __launch_bounds__(256) void k_wbp_colfft(const float2* __restrict__ tbuf,
+                                                    float2* __restrict__ ubuf, int vol_x,
+                                                    int vol_z, int kb, FftDesc fz2, FftDesc fz1)
+{
+    extern __shared__ float2 lds[];
+    const int nz2 = 2 * vol_z, kxn = vol_x / 2 + 1;
+    float2* a = lds;
+    float2* b = lds + kb * nz2;
+    const int k0 = blockIdx.x * kb, ys = blockIdx.y;
+    const int nk = min(kb, kxn - k0);
+    const float2* src = tbuf + (size_t)ys * nz2 * kxn;
+    for (int i = threadIdx.x; i < kb * nz2; i += blockDim.x) {
+        int col = i % kb, z = i / kb;
+        a[col * nz2 + z] = col < nk ? src[(size_t)z * kxn + k0 + col] : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    float2* r = lds_fft(a, b, fz2, kb, -1.0f, threadIdx.x, blockDim.x);
+    float2* o = (r == a) ? b : a;
+    const float norm = (float)(1.0 / ((double)(2 * vol_x) * (double)nz2));
+    for (int i = threadIdx.x; i < kb * vol_z; i += blockDim.x) {
+        int col = i / vol_z, j = i % vol_z;
+        int srcz = j <= vol_z / 2 ? j : j + vol_z;
+        float2 v = r[col * nz2 + srcz];
+        o[col * vol_z + j] = make_float2(v.x * norm, v.y * norm);
+    }
+    __syncthreads();
+    float2* q = lds_fft(o, r, fz1, kb, 1.0f, threadIdx.x, blockDim.x);
+    float2* dst = ubuf + (size_t)ys * vol_z * kxn;
+    for (int i = threadIdx.x; i < kb * vol_z; i += blockDim.x) {
+        int col = i % kb, z = i / kb;
+        if (col < nk) dst[(size_t)z * kxn + k0 + col] = q[col * vol_z + z];
+    }
+}
+
+__device__ __forceinline__ size_t out_index_w(int layout, int x, int y, int z, int vol_x,
+                                              int vol_y, int vol_z)
+{
+    switch (layout) {
+    case ATR_LAYOUT_XYZ: return ((size_t)z * vol_y + y) * vol_x + x;
+    case ATR_LAYOUT_XYZ_NOFLIP: return ((size_t)(vol_z - 1 - z) * vol_y + y) * vol_x + x;
+    case ATR_LAYOUT_SLICE: return ((size_t)y * vol_z + z) * vol_x + x;
+    default: return ((size_t)y * vol_z + (vol_z - 1 - z)) * vol_x + x;
+    }
+}
+
+// One workgroup per (slice, pair of output rows): the unnormalised C2R of
+// length VolX of two half spectra, packed as one complex inverse FFT.  The
+// imaginary parts of the DC and Nyquist bins are ignored (numpy irfft).
+__global__ __launch_bounds__(256) void k_wbp_rowc2r(const float2* __restrict__ ubuf, int vol_x,
+                                                    int vol_y, int vol_z, int y0, FftDesc fx,
+                                                    int layout, float* __restrict__ out)
+{
+    extern __shared__ float2 lds[];
+    float2* a = lds;
+    float2* b = lds + vol_x;
+    const int kxn = vol_x / 2 + 1, half = vol_x / 2;
+    const int zp = blockIdx.x, ys = blockIdx.y, y = y0 + ys;
+    const int za = 2 * zp, zb = za + 1;
+    const bool has_b = zb < vol_z;
+    const float2* ha = ubuf + ((size_t)ys * vol_z + za) * kxn;
+    const float2* hb = ubuf + ((size_t)ys * vol_z + (has_b ? zb : za)) * kxn;
+    for (int k = threadIdx.x; k < vol_x; k += blockDim.x) {
+        float2 va, vb;
+        if (k == 0 || k == half) {
+            va = make_float2(ha[k].x, 0.0f);
+            vb = make_float2(hb[k].x, 0.0f);
+        } else if (k < half) {
+            va = ha[k];
+            vb = hb[k];
+        } else {
+            float2 u = ha[vol_x - k], w = hb[vol_x - k];
+            va = make_float2(u.x, -u.y);
+            vb = make_float2(w.x, -w.y);
+        }
+        if (!has_b) vb = make_float2(0.0f, 0.0f);
+        a[k] = make_float2(va.x - vb.y, va.y + vb.x);
+    }
+    __syncthreads();
+    float2* r = lds_fft(a, b, fx, 1, 1.0f, threadIdx.x, blockDim.x);
+    for (int x = threadIdx.x; x < vol_x; x += blockDim.x) {
+        out[out_index_w(layout, x, y, za, vol_x, vol_y, vol_z)] = r[x].x;
+        if (has_b) out[out_index_w(layout, x, y, zb, vol_x, vol_y, vol_z)] = r[x].y;
+    }
+}
+
+// ------------------------------------------------------------------------
+template <typename S>
+static void launch_filter_t(const Plan& p, const float* in, hipStream_t s)
+{
+    const float* src = p.prestage ? p.d_aligned : in;
+    size_t ts = p.prestage ? (size_t)p.aln_x * p.aln_y : p.in_tilt_stride;
+    size_t rs = p.prestage ? (size_t)p.aln_x : p.in_row_stride;
+    dim3 grid((p.vol_y + 1) / 2, p.n);
+    size_t shm = 2 * (size_t)p.proj * sizeof(float2);
+    hipLaunchKernelGGL(k_filter<S>, grid, dim3(256), shm, s, src, ts, rs, p.fill ? 1 : 0,
+                       p.d_fill_aln, (const double2*)p.d_tilt_stats, p.d_cs, p.n, p.vol_y, p.proj,
+                       p.vol_z, p.fft_proj, p.d_filter, (S*)p.d_sino, p.sino_stride, p.d_sflag);
+}
+
+void launch_filter_wbp(const Plan& p, const float* in, hipStream_t s)
+{
+    if (p.storage == ATR_STORAGE_F16) launch_filter_t<_Float16>(p, in, s);
+    else if (p.storage == ATR_STORAGE_BF16) launch_filter_t<bf16_t>(p, in, s);
+    else launch_filter_t<float>(p, in, s);
+}
+
+template <typename S>
+static void launch_wbp_bp_t(const Plan& p, int y0, int ny, hipStream_t s)
+{
+    dim3 grid(p.vol_z, ny);
+    size_t shm = 2 * (size_t)(2 * p.vol_x) * sizeof(float2);
+    hipLaunchKernelGGL(k_wbp_bp_rowfft<S>, grid, dim3(256), shm, s, (const S*)p.d_sino, p.n,
+                       p.sino_stride, p.proj, p.d_cs, p.d_excl, p.d_wbp_lohi, p.d_sflag, p.vol_x,
+                       p.vol_z, y0, p.fft_x2, p.d_wbp_t);
+}
+
+void launch_wbp_bp_rowfft(const Plan& p, int y0, int ny, hipStream_t s)
+{
+    if (p.storage == ATR_STORAGE_F16) launch_wbp_bp_t<_Float16>(p, y0, ny, s);
+    else if (p.storage == ATR_STORAGE_BF16) launch_wbp_bp_t<bf16_t>(p, y0, ny, s);
+    else launch_wbp_bp_t<float>(p, y0, ny, s);
+}
+
+int wbp_col_block(int vol_z)
+{
+    int kb = 2048 / (2 * vol_z);
+    if (kb < 1) kb = 1;
+    if (kb > 16) kb = 16;
+    return kb;
+}
+
+void launch_wbp_colfft(const Plan& p, int y0, int ny, hipStream_t s)
+{
+    (void)y0;
+    int kb = wbp_col_block(p.vol_z);
+    int kxn = p.vol_x / 2 + 1;
+    dim3 grid((kxn + kb - 1) / kb, ny);
+    size_t shm = 2 * (size_t)kb * (2 * p.vol_z) * sizeof(float2);
+    hipLaunchKernelGGL(k_wbp_colfft, grid, dim3(256), shm, s, p.d_wbp_t, p.d_wbp_u, p.vol_x,
+                       p.vol_z, kb, p.fft_z2, p.fft_z);
+}
+
+void launch_wbp_rowc2r(const Plan& p, int y0, int ny, float* out, hipStream_t s)
+{
+    dim3 grid((p.vol_z + 1) / 2, ny);
+    size_t shm = 2 * (size_t)p.vol_x * sizeof(float2);
+    hipLaunchKernelGGL(k_wbp_rowc2r, grid, dim3(256), shm, s, p.d_wbp_u, p.vol_x, p.vol_y,
+                       p.vol_z, y0, p.fft_x, p.layout, out);
+}
+
+}  // namespace atr

@@ -1,0 +1,52 @@
+// storage.h -- storage types of the weighted projections (FP32 / FP16 /
+// BF16) and their FP32 load/store.  Only storage changes; every sum is FP32.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace atr {
+
+struct bf16_t {
+    uint16_t bits;
+};
+
+template <typename S>
+__device__ __forceinline__ float ld(const S* p);
+template <>
+__device__ __forceinline__ float ld<float>(const float* p) { return *p; }
+template <>
+__device__ __forceinline__ float ld<_Float16>(const _Float16* p) { return (float)*p; }
+template <>
+__device__ __forceinline__ float ld<bf16_t>(const bf16_t* p)
+{
+    return __uint_as_float((uint32_t)p->bits << 16);
+}
+
+template <typename S>
+__device__ __forceinline__ void st(S* p, float v);
+template <>
+__device__ __forceinline__ void st<float>(float* p, float v) { *p = v; }
+template <>
+__device__ __forceinline__ void st<_Float16>(_Float16* p, float v)
+{
+    // Sentinels (-1e30) do not fit FP16; they saturate to -65504, which the
+    // kernels still classify as "empty" through the storage-aware threshold.
+    *p = (_Float16)fmaxf(v, -65504.0f);
+}
+template <>
+__device__ __forceinline__ void st<bf16_t>(bf16_t* p, float v)
+{
+    uint32_t b = __float_as_uint(v);
+    b += 0x7FFFu + ((b >> 16) & 1u);  // round to nearest even
+    p->bits = (uint16_t)(b >> 16);
+}
+
+// Threshold below which a stored value counts as the reference's "empty"
+// (<= -1e10).  FP16 cannot hold -1e10, so its sentinel is the most negative
+// finite half; real data never gets there after weighting by cos/VolZ.
+template <typename S>
+__device__ __forceinline__ float empty_threshold() { return -1e10f; }
+template <>
+__device__ __forceinline__ float empty_threshold<_Float16>() { return -65000.0f; }
+
+}  // namespace atr

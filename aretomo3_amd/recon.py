@@ -1,0 +1,273 @@
+"""Host-side mirror of AreTomo3's reconstruction operators over the C ABI.
+
+The class and argument names follow the reference so that callers and tests
+read like its own code:
+
+* :class:`TomoWbp`, :class:`TomoSart` -- the per-slice operators
+  ``CTomoWbp``/``CTomoSart`` (AreTomo/Recon/CReconInc.h:173-236): ``setup``
+  once, then ``do_it(pad_sinogram, vol_xz, stream)`` on device buffers.
+* :class:`DoWbpRecon`, :class:`DoSartRecon` -- the per-series drivers
+  ``CDoWbpRecon``/``CDoSartRecon`` (AreTomo/Recon/CDoWbpRecon.cpp:28-47,
+  CDoSartRecon.cpp:30-57): host tilt series in, host volume out in the
+  reference's XZY frame layout (z flipped).
+* :class:`Plan` -- the general plan (pre-stage, storage, layouts, SART
+  options) behind all of them.
+
+Every computation runs in libatr_recon.so on the GPU; this module only moves
+arguments.  Errors surface as :class:`AtrError` carrying the C status.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import AtrError, check, lib
+
+
+def _fptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class DeviceArray:
+    """A device allocation made through the library (no other runtime)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().atr_malloc(ctypes.byref(p), max(self.nbytes, 4)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.nbytes)
+        if arr.nbytes:
+            check(lib().atr_memcpy_h2d(d.ptr, arr.ctypes.data, arr.nbytes))
+        return d
+
+    def to_numpy(self, shape, dtype=np.float32):
+        out = np.empty(shape, dtype)
+        if out.nbytes:
+            check(lib().atr_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(lib().atr_memcpy_h2d(self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def free(self):
+        if self.ptr:
+            check(lib().atr_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    st = lib().atr_device_count(ctypes.byref(n))
+    return n.value if st == L.ATR_OK else 0
+
+
+def synchronize():
+    check(lib().atr_synchronize())
+
+
+class Plan:
+    """One reconstruction plan (atr_plan_create).  Arrays given here are
+    copied into the descriptor; the plan keeps them alive."""
+
+    def __init__(self, proj_x, proj_y, tilts, vol_z, method=L.METHOD_SART, *, tilt_axis=None,
+                 shifts=None, random_fill=True, exclude=None, sart_iters=20, sart_subsets=1,
+                 sart_start_tilt=0, sart_n_tilts=0, relax=-1.0,
+                 subset_order=L.SUBSETS_CONTIGUOUS, positivity=True, storage=L.STORAGE_F32,
+                 layout=L.LAYOUT_XZY, fill_seed=0, tilt_stride=0, row_stride=0):
+        self._tilts = np.ascontiguousarray(tilts, np.float32)
+        n = len(self._tilts)
+        self._axis = None
+        if tilt_axis is not None:
+            ax = np.asarray(tilt_axis, np.float32)
+            self._axis = np.ascontiguousarray(np.broadcast_to(ax, (n,)), np.float32)
+        self._shifts = None if shifts is None else np.ascontiguousarray(shifts, np.float32).reshape(-1)
+        self._excl = None if exclude is None else np.ascontiguousarray(exclude, np.uint8)
+        d = L.SeriesDesc()
+        lib().atr_series_desc_init(ctypes.byref(d))
+        d.proj_x, d.proj_y, d.n_tilts = int(proj_x), int(proj_y), n
+        d.tilt_stride, d.row_stride = int(tilt_stride), int(row_stride)
+        d.tilt_deg = self._tilts.ctypes.data
+        d.tilt_axis_deg = None if self._axis is None else self._axis.ctypes.data
+        d.shift_xy = None if self._shifts is None else self._shifts.ctypes.data
+        d.random_fill = 1 if random_fill else 0
+        d.exclude = None if self._excl is None else self._excl.ctypes.data
+        d.vol_z = int(vol_z)
+        d.method = int(method)
+        d.sart_iters, d.sart_subsets = int(sart_iters), int(sart_subsets)
+        d.sart_start_tilt, d.sart_n_tilts = int(sart_start_tilt), int(sart_n_tilts)
+        d.sart_relax = float(relax)
+        d.subset_order, d.positivity = int(subset_order), 1 if positivity else 0
+        d.storage, d.out_layout = int(storage), int(layout)
+        d.fill_seed = int(fill_seed)
+        self.desc = d
+        self.layout = int(layout)
+        h = ctypes.c_void_p()
+        check(lib().atr_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self.handle = h.value
+        vx, vy, vz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().atr_plan_get_volume_size(self.handle, ctypes.byref(vx), ctypes.byref(vy),
+                                             ctypes.byref(vz)))
+        self.vol_x, self.vol_y, self.vol_z = vx.value, vy.value, vz.value
+        self.n_tilts = n
+        self.input_elems = (d.tilt_stride or (d.row_stride or proj_x) * proj_y) * (n - 1) + \
+            (d.row_stride or proj_x) * (proj_y - 1) + proj_x
+
+    @staticmethod
+    def estimate_bytes(**kw):
+        """Device bytes a plan with these arguments would allocate."""
+        p = Plan.__new__(Plan)
+        tilts = np.ascontiguousarray(kw["tilts"], np.float32)
+        d = L.SeriesDesc()
+        lib().atr_series_desc_init(ctypes.byref(d))
+        d.proj_x, d.proj_y, d.n_tilts = kw["proj_x"], kw["proj_y"], len(tilts)
+        d.tilt_deg = tilts.ctypes.data
+        d.vol_z, d.method = kw["vol_z"], kw.get("method", L.METHOD_SART)
+        d.sart_iters, d.sart_subsets = kw.get("sart_iters", 20), kw.get("sart_subsets", 1)
+        d.storage = kw.get("storage", L.STORAGE_F32)
+        axis = None
+        if kw.get("tilt_axis") is not None:
+            axis = np.full(len(tilts), kw["tilt_axis"], np.float32)
+            d.tilt_axis_deg = axis.ctypes.data
+        b = ctypes.c_size_t()
+        check(lib().atr_estimate_device_bytes(ctypes.byref(d), ctypes.byref(b)))
+        del p
+        return b.value
+
+    @property
+    def volume_shape(self):
+        if self.layout in (L.LAYOUT_XYZ, L.LAYOUT_XYZ_NOFLIP):
+            return (self.vol_z, self.vol_y, self.vol_x)
+        return (self.vol_y, self.vol_z, self.vol_x)
+
+    @property
+    def device_bytes(self):
+        return int(lib().atr_plan_device_bytes(self.handle))
+
+    def execute(self, proj_dev, vol_dev, accumulate=False, stream=None):
+        """Stream-ordered run on device pointers (ints or DeviceArray)."""
+        pp = proj_dev.ptr if isinstance(proj_dev, DeviceArray) else proj_dev
+        vp = vol_dev.ptr if isinstance(vol_dev, DeviceArray) else vol_dev
+        check(lib().atr_execute(self.handle, pp, vp, 1 if accumulate else 0, stream))
+
+    def execute_host(self, proj):
+        """Host numpy in, host numpy volume out (CDo*Recon::DoIt)."""
+        proj = np.ascontiguousarray(proj, np.float32).reshape(-1)
+        assert proj.size >= self.input_elems
+        out = np.empty(self.volume_shape, np.float32)
+        check(lib().atr_execute_host(self.handle, proj.ctypes.data, out.ctypes.data))
+        return out
+
+    def set_profiling(self, on=True):
+        check(lib().atr_plan_set_profiling(self.handle, 1 if on else 0))
+
+    def kernel_stats(self, kernel):
+        n, ms = ctypes.c_int(), ctypes.c_double()
+        check(lib().atr_plan_kernel_stats(self.handle, kernel, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def reset_stats(self):
+        check(lib().atr_plan_reset_stats(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().atr_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------------------------
+# per-slice operators (CTomoWbp / CTomoSart)
+# --------------------------------------------------------------------------
+class _TomoBase:
+    def _make(self, vol_x, vol_z, proj_x, tilts, method, **kw):
+        if vol_x != proj_x // 2 * 2:
+            raise AtrError(1, f"iVolX must be ProjX/2*2 = {proj_x // 2 * 2}")
+        self.proj_x = int(proj_x)
+        self.pad_x = (self.proj_x // 2 + 1) * 2
+        self.plan = Plan(self.proj_x, 1, tilts, vol_z, method, tilt_stride=self.pad_x,
+                         row_stride=self.pad_x, layout=L.LAYOUT_SLICE, **kw)
+
+    def clean(self):
+        if getattr(self, "plan", None) is not None:
+            self.plan.close()
+            self.plan = None
+
+
+class TomoWbp(_TomoBase):
+    """CTomoWbp (AreTomo/Recon/CReconInc.h:173-192)."""
+
+    def setup(self, vol_x, vol_z, proj_x, tilts, exclude=None):
+        self.clean()
+        self._make(vol_x, vol_z, proj_x, tilts, L.METHOD_WBP, exclude=exclude)
+
+    def do_it(self, pad_sinogram, vol_xz, stream=None):
+        """pad_sinogram: device [n_tilts][(ProjX/2+1)*2]; vol_xz: device [VolZ][VolX]."""
+        self.plan.execute(pad_sinogram, vol_xz, False, stream)
+
+
+class TomoSart(_TomoBase):
+    """CTomoSart (AreTomo/Recon/CReconInc.h:194-236).  do_it continues from
+    the volume already in vol_xz, as the reference does."""
+
+    def setup(self, vol_x, vol_z, num_subsets, num_iters, proj_x, tilts, start_tilt, num_tilts,
+              relax=-1.0, subset_order=L.SUBSETS_CONTIGUOUS, positivity=True, fill_seed=0,
+              exclude=None, storage=L.STORAGE_F32):
+        self.clean()
+        self._make(vol_x, vol_z, proj_x, tilts, L.METHOD_SART, sart_iters=num_iters,
+                   sart_subsets=num_subsets, sart_start_tilt=start_tilt, sart_n_tilts=num_tilts,
+                   relax=relax, subset_order=subset_order, positivity=positivity,
+                   fill_seed=fill_seed, exclude=exclude, storage=storage)
+
+    def do_it(self, pad_sinogram, vol_xz, stream=None):
+        self.plan.execute(pad_sinogram, vol_xz, True, stream)
+
+
+# --------------------------------------------------------------------------
+# per-series drivers (CDoWbpRecon / CDoSartRecon)
+# --------------------------------------------------------------------------
+class DoWbpRecon:
+    """CDoWbpRecon::DoIt(pTiltSeries, pAlignParam, iVolZ) -> XZY volume."""
+
+    def do_it(self, tilt_series, tilts, vol_z, **kw):
+        ts = np.ascontiguousarray(tilt_series, np.float32)
+        n, py, px = ts.shape
+        plan = Plan(px, py, tilts, vol_z, L.METHOD_WBP, **kw)
+        try:
+            return plan.execute_host(ts)
+        finally:
+            plan.close()
+
+
+class DoSartRecon:
+    """CDoSartRecon::DoIt(pTiltSeries, pAlignParam, iStartTilt, iNumTilts,
+    iVolZ, iIterations, iNumSubsets) -> XZY volume."""
+
+    def do_it(self, tilt_series, tilts, start_tilt, num_tilts, vol_z, iterations, num_subsets,
+              **kw):
+        ts = np.ascontiguousarray(tilt_series, np.float32)
+        n, py, px = ts.shape
+        plan = Plan(px, py, tilts, vol_z, L.METHOD_SART, sart_iters=iterations,
+                    sart_subsets=num_subsets, sart_start_tilt=start_tilt,
+                    sart_n_tilts=num_tilts, **kw)
+        try:
+            return plan.execute_host(ts)
+        finally:
+            plan.close()

@@ -1,0 +1,191 @@
+/*
+ * aretomo_recon.h -- C ABI of the MI355X (gfx950) reconstruction library
+ * libatr_recon.so: AreTomo3's tomographic reconstruction hot path
+ * (weighting + ramp/Hamming filter, weighted back-projection, SART) plus the
+ * stack-correction pre-stage that feeds it.
+ *
+ * Reference interfaces replaced (paths relative to the AreTomo3 tree):
+ *   atr_plan_create / atr_execute with ATR_LAYOUT_SLICE and proj_y == 1
+ *       -> CTomoWbp::Setup/DoIt   AreTomo/Recon/CReconInc.h:173-192
+ *       -> CTomoSart::Setup/DoIt  AreTomo/Recon/CReconInc.h:194-236
+ *   atr_plan_create / atr_execute(_host) on a whole series
+ *       -> CDoWbpRecon::DoIt      AreTomo/Recon/CReconInc.h:259-279,
+ *                                 AreTomo/Recon/CDoWbpRecon.cpp:28-47
+ *       -> CDoSartRecon::DoIt     AreTomo/Recon/CReconInc.h:281-310,
+ *                                 AreTomo/Recon/CDoSartRecon.cpp:30-57
+ *   tilt_axis_deg / shift_xy in the descriptor (pre-stage inside the plan)
+ *       -> CCorrTomoStack::Set1..Set4 + DoIt
+ *                                 AreTomo/Correct/CCorrTomoStack.cpp:75-222
+ *   ATR_LAYOUT_XYZ / ATR_LAYOUT_XYZ_NOFLIP
+ *       -> CTiltSeries::FlipVol   DataUtil/CTiltSeries.cpp:286-305
+ *
+ * Conventions
+ *   - All entry points return atr_status and never abort; the message of the
+ *     last failure on the calling thread is atr_last_error_string().
+ *   - A plan owns its device scratch and geometry tables and holds no global
+ *     device state, so many plans may run concurrently on one device on
+ *     different streams.  One plan must not be executed concurrently with
+ *     itself.
+ *   - Device pointers are HIP device pointers; `stream` is a hipStream_t
+ *     passed as void* (NULL = the default stream).  Work is stream ordered.
+ */
+#ifndef ARETOMO_RECON_H
+#define ARETOMO_RECON_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum atr_status {
+    ATR_OK = 0,
+    ATR_ERR_INVALID_ARG = 1,
+    ATR_ERR_OUT_OF_MEMORY = 2,
+    ATR_ERR_HIP = 3,
+    ATR_ERR_UNSUPPORTED_SIZE = 4
+} atr_status;
+
+typedef enum atr_method { ATR_METHOD_WBP = 0, ATR_METHOD_SART = 1 } atr_method;
+
+/* Storage type of the weighted projections on the device.  Voxel sums are
+ * always accumulated in FP32. */
+typedef enum atr_storage {
+    ATR_STORAGE_F32 = 0,
+    ATR_STORAGE_F16 = 1,
+    ATR_STORAGE_BF16 = 2
+} atr_storage;
+
+/* SART subset composition.  CONTIGUOUS = CSplitItems blocks
+ * (AreTomo/Util/CSplitItems.cpp:25-48), the reference behaviour.
+ * INTERLEAVED = subset s holds tilts start+s, start+s+n_subsets, ... */
+typedef enum atr_subset_order {
+    ATR_SUBSETS_CONTIGUOUS = 0,
+    ATR_SUBSETS_INTERLEAVED = 1
+} atr_subset_order;
+
+/* Output volume layout.
+ *   XZY          frame y = slice y, rows z flipped (row VolZ-1-z): the
+ *                CTiltSeries returned by CDo*Recon (CDoWbpRecon.cpp:107-123)
+ *   XYZ          -FlipVol 1: frame z (physical), rows y
+ *   XYZ_NOFLIP   -FlipVol 2: frame VolZ-1-z, rows y
+ *   SLICE        frame y, rows z unflipped: the gfVolXZ of CTomo*::DoIt */
+typedef enum atr_layout {
+    ATR_LAYOUT_XZY = 0,
+    ATR_LAYOUT_XYZ = 1,
+    ATR_LAYOUT_XYZ_NOFLIP = 2,
+    ATR_LAYOUT_SLICE = 3
+} atr_layout;
+
+typedef struct atr_series_desc {
+    /* input projections: n_tilts images of proj_x * proj_y floats */
+    int proj_x;
+    int proj_y;
+    int n_tilts;
+    /* element strides of the input (0 = dense [tilt][y][x]); the padded
+     * sinogram of CTomo*::DoIt is proj_y = 1, tilt_stride = (proj_x/2+1)*2 */
+    size_t tilt_stride;
+    size_t row_stride;
+    const float* tilt_deg;      /* [n_tilts]   CAlignParam::GetTilts */
+    /* pre-stage (CCorrTomoStack): NULL tilt_axis_deg = input already aligned */
+    const float* tilt_axis_deg; /* [n_tilts]   CAlignParam::GetTiltAxis */
+    const float* shift_xy;      /* [2*n_tilts] CAlignParam::GetShift, NULL = 0 */
+    int random_fill;            /* fill empty pixels (forced on, as -AtBin FFT crop) */
+    const unsigned char* exclude; /* [n_tilts] gbNoProjs (CTomoBase.cpp:48-51), NULL = none */
+    int vol_z;
+    int method;                 /* atr_method */
+    /* SART (CTomoSart::Setup, CDoSartRecon::DoIt) */
+    int sart_iters;
+    int sart_subsets;
+    int sart_start_tilt;
+    int sart_n_tilts;           /* <= 0: n_tilts - sart_start_tilt */
+    float sart_relax;           /* <= 0: 1/sart_subsets (CTomoSart.cpp:71) */
+    int subset_order;           /* atr_subset_order */
+    int positivity;             /* 1 = clamp at 0 as GBackProj.cu:56 (reference) */
+    int storage;                /* atr_storage */
+    int out_layout;             /* atr_layout */
+    uint64_t fill_seed;         /* seeds the empty-fill index tables */
+} atr_series_desc;
+
+typedef struct atr_plan atr_plan;
+
+/* Fills *desc with the reference defaults (SART 20 iterations, positivity
+ * on, contiguous subsets, relax 1/subsets, FP32, XZY, random fill on). */
+void atr_series_desc_init(atr_series_desc* desc);
+
+/* Validates the descriptor, computes the FP64 geometry tables and allocates
+ * the device scratch on the current device. */
+atr_status atr_plan_create(const atr_series_desc* desc, atr_plan** plan);
+
+/* Size queries: the output volume holds vol_x * vol_y * vol_z floats. */
+atr_status atr_plan_get_volume_size(const atr_plan* plan, int* vol_x, int* vol_y,
+                                    int* vol_z);
+/* Device bytes owned by the plan (tables + scratch). */
+size_t atr_plan_device_bytes(const atr_plan* plan);
+/* Device bytes a plan for *desc would allocate (serving admission). */
+atr_status atr_estimate_device_bytes(const atr_series_desc* desc, size_t* bytes);
+
+/* Runs the reconstruction on `stream`.  proj_dev: device input laid out as
+ * described in the plan's descriptor; vol_dev: device output in out_layout.
+ * accumulate != 0 (SART only): the SART starts from the volume already in
+ * vol_dev (ATR_LAYOUT_SLICE), as CTomoSart::DoIt does with gfVolXZ;
+ * otherwise from zeros, as CDoSartRecon does (CDoSartRecon.cpp:271-279). */
+atr_status atr_execute(atr_plan* plan, const float* proj_dev, float* vol_dev,
+                       int accumulate, void* stream);
+
+/* Host-memory variant (drop-in for CDoWbpRecon/CDoSartRecon::DoIt): copies
+ * proj_host in, reconstructs, copies the volume out, synchronises. */
+atr_status atr_execute_host(atr_plan* plan, const float* proj_host, float* vol_host);
+
+/* Per-kernel timing of the last executions (hipEvents on the plan stream);
+ * enabled with atr_plan_set_profiling(plan, 1).  kernel: 0 = SART BP,
+ * 1 = FP+diff, 2 = weighting/filter, 3 = WBP BP (2x grid + row FFT),
+ * 4 = pre-stage, 5 = output/fill, 6 = WBP z-decimation, 7 = WBP row C2R. */
+atr_status atr_plan_set_profiling(atr_plan* plan, int enable);
+atr_status atr_plan_kernel_stats(const atr_plan* plan, int kernel, int* launches,
+                                 double* total_ms);
+atr_status atr_plan_reset_stats(atr_plan* plan);
+
+void atr_plan_destroy(atr_plan* plan);
+
+const char* atr_last_error_string(void);
+const char* atr_version(void);
+
+/* ---- synthetic tilt series (seeded ellipsoid phantoms) -----------------
+ * Analytic parallel-beam line integrals of n_ellipsoids random ellipsoids
+ * (centres uniform in the volume, semi-axes uniform [axis_min, axis_max],
+ * uniform random orientation, density uniform [0.5, 1], additive), tilted
+ * about y, optionally mapped to raw images with a tilt axis and per-tilt
+ * shifts (the inverse of the pre-stage), plus Gaussian noise of
+ * sigma = noise_frac * mean.  Output [n_tilts][proj_y][proj_x] host floats.
+ * shift_xy_out (may be NULL) receives the per-tilt shifts drawn uniformly in
+ * [-max_shift, max_shift]. */
+atr_status atr_phantom_tilt_series(uint64_t seed, int proj_x, int proj_y, int n_tilts,
+                                   const float* tilt_deg, int vol_z, int n_ellipsoids,
+                                   float axis_min, float axis_max, float tilt_axis_deg,
+                                   float max_shift, float noise_frac, float* shift_xy_out,
+                                   float* proj_out, int n_threads);
+
+/* ---- device helpers (plumbing for hosts without a HIP binding) --------- */
+atr_status atr_device_count(int* count);
+atr_status atr_set_device(int device);
+atr_status atr_malloc(void** ptr, size_t bytes);
+atr_status atr_free(void* ptr);
+atr_status atr_memcpy_h2d(void* dst, const void* src, size_t bytes);
+atr_status atr_memcpy_d2h(void* dst, const void* src, size_t bytes);
+atr_status atr_memset(void* ptr, int value, size_t bytes);
+atr_status atr_synchronize(void);
+atr_status atr_stream_create(void** stream);
+atr_status atr_stream_destroy(void* stream);
+atr_status atr_stream_synchronize(void* stream);
+atr_status atr_mem_info(size_t* free_bytes, size_t* total_bytes);
+/* Wall time of the stream between two markers, milliseconds. */
+atr_status atr_event_create(void** ev);
+atr_status atr_event_destroy(void* ev);
+atr_status atr_event_record(void* ev, void* stream);
+atr_status atr_event_elapsed_ms(void* start, void* stop, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,279 @@
+"""FP64 CPU oracle for AreTomo3's reconstruction hot path (ctypes over liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the reported CPU
+baseline.  Only ``tests/``, ``__graft_entry__.smoke()`` and the cpu_baseline
+leg of ``bench.py`` import this package.  The product (``aretomo3_amd``)
+never imports, links or calls it.
+
+Each wrapper names the C function in ``oracle.c``; the C functions cite the
+reference file:line they restate.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_c_int, _c_u64, _c_f, _c_d, _c_vp = (ctypes.c_int, ctypes.c_uint64, ctypes.c_float,
+                                      ctypes.c_double, ctypes.c_void_p)
+
+
+def build():
+    """Compile liboracle.so from oracle.c (gcc, OpenMP)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "orc_splitmix64": (ctypes.c_uint64, [_c_u64]),
+            "orc_fill_table": (None, [_c_u64, _c_int, _ip]),
+            "orc_cos_sin": (None, [_fp, _c_int, _fp]),
+            "orc_split_items": (None, [_c_int, _c_int, _ip, _ip]),
+            "orc_subset_tilts": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _ip]),
+            "orc_fft": (None, [_dp, _dp, _c_int, _c_int]),
+            "orc_aligned_size": (None, [_c_int, _c_int, _c_f, ctypes.POINTER(_c_int),
+                                        ctypes.POINTER(_c_int)]),
+            "orc_weight_projs": (None, [_dp, _c_int, _c_int, _c_int, _fp, _c_int]),
+            "orc_rweight": (None, [_dp, _c_int, _c_int, _c_int]),
+            "orc_back_proj": (None, [_dp, _c_int, _c_int, _fp, _c_vp, _ip, _c_int, _c_int,
+                                     _c_d, _dp, _c_int, _c_int]),
+            "orc_for_proj": (None, [_dp, _c_int, _c_int, _fp, _ip, _c_int, _c_int, _c_int, _dp]),
+            "orc_diff_proj": (None, [_dp, _dp, _ip, _c_int, _c_int, _c_int]),
+            "orc_fill_empty": (None, [_dp, _c_int, _c_int, _c_int, _ip]),
+            "orc_back_proj_wbp": (None, [_dp, _c_int, _c_int, _fp, _c_vp, _c_int, _dp,
+                                         _c_int, _c_int]),
+            "orc_wbp_crop": (None, [_dp, _c_int, _c_int, _dp]),
+            "orc_tomo_sart": (None, [_dp, _c_int, _c_int, _c_int, _fp, _c_vp, _dp, _c_int,
+                                     _c_int, _c_int, _c_int, _c_int, _c_int, _c_d, _c_int,
+                                     _c_int, _ip]),
+            "orc_tomo_wbp": (None, [_dp, _c_int, _c_int, _c_int, _fp, _c_vp, _dp, _c_int,
+                                    _c_int]),
+            "orc_corr_proj": (None, [_fp, _c_int, _c_int, _c_f, _c_f, _c_f, _dp, _c_int,
+                                     _c_int, _c_int, _c_int, _c_vp]),
+            "orc_prestage": (None, [_fp, _c_int, _c_int, _c_int, _fp, _c_vp, _c_int, _c_u64,
+                                    _dp, _c_int, _c_int]),
+            "orc_recon_slices": (None, [_dp, _c_int, _c_int, _c_int, _fp, _c_vp, _c_int,
+                                        _c_int, _c_int, _c_int, _c_int, _c_int, _c_d, _c_int,
+                                        _c_int, _c_u64, _ip, _c_int, _dp]),
+            "orc_flip_vol": (None, [_dp, _c_int, _c_int, _c_int, _c_int, _dp]),
+            "orc_max_threads": (_c_int, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _mask(no_proj):
+    if no_proj is None:
+        return None, None
+    m = np.ascontiguousarray(no_proj, dtype=np.uint8)
+    return m, m.ctypes.data_as(ctypes.c_void_p)
+
+
+def splitmix64(x):
+    return int(lib().orc_splitmix64(x))
+
+
+def fill_table(seed, n):
+    t = np.empty(n, np.int32)
+    lib().orc_fill_table(seed, n, t)
+    return t
+
+
+def cos_sin(tilts):
+    tilts = _f32(tilts)
+    out = np.empty(2 * len(tilts), np.float32)
+    lib().orc_cos_sin(tilts, len(tilts), out)
+    return out
+
+
+def split_items(n_items, n_splits):
+    st = np.empty(n_splits, np.int32)
+    sz = np.empty(n_splits, np.int32)
+    lib().orc_split_items(n_items, n_splits, st, sz)
+    return st, sz
+
+
+def subset_tilts(start, n_tilts, n_subsets, order, subset):
+    buf = np.empty(n_tilts + 1, np.int32)
+    n = lib().orc_subset_tilts(start, n_tilts, n_subsets, order, subset, buf)
+    return buf[:n].copy()
+
+
+def fft(x, sign=-1):
+    x = np.asarray(x, np.complex128)
+    re, im = _f64(x.real), _f64(x.imag)
+    lib().orc_fft(re, im, len(x), sign)
+    return re + 1j * im
+
+
+def aligned_size(raw_x, raw_y, axis):
+    ax, ay = _c_int(), _c_int()
+    lib().orc_aligned_size(raw_x, raw_y, axis, ctypes.byref(ax), ctypes.byref(ay))
+    return ax.value, ay.value
+
+
+def weight_projs(sino, proj_x, cs, vol_z):
+    s = _f64(sino).copy()
+    lib().orc_weight_projs(s, s.shape[0], s.shape[1], proj_x, _f32(cs), vol_z)
+    return s
+
+
+def rweight(sino, fft_size):
+    s = _f64(sino).copy()
+    lib().orc_rweight(s, s.shape[0], s.shape[1], fft_size)
+    return s
+
+
+def back_proj(sino, proj_x, cs, tilts, vol, positivity=1, relax=1.0, no_proj=None):
+    s = _f64(sino)
+    v = _f64(vol).copy()
+    t = _i32(tilts)
+    m, mp = _mask(no_proj)
+    lib().orc_back_proj(s, s.shape[1], proj_x, _f32(cs), mp, t, len(t), positivity, relax,
+                        v, v.shape[1], v.shape[0])
+    return v
+
+
+def for_proj(vol, cs, tilts, proj_x, pad_x, n_tilts):
+    v = _f64(vol)
+    t = _i32(tilts)
+    fwd = np.zeros((n_tilts, pad_x), np.float64)
+    lib().orc_for_proj(v, v.shape[1], v.shape[0], _f32(cs), t, len(t), proj_x, pad_x, fwd)
+    return fwd
+
+
+def diff_proj(raw, fwd, tilts, proj_x):
+    f = _f64(fwd).copy()
+    t = _i32(tilts)
+    lib().orc_diff_proj(_f64(raw), f, t, len(t), proj_x, f.shape[1])
+    return f
+
+
+def fill_empty(img, w, table):
+    im = _f64(img).copy()
+    lib().orc_fill_empty(im, w, im.shape[0], im.shape[1], _i32(table))
+    return im
+
+
+def back_proj_wbp(sino, proj_x, cs, vol_x2, vol_z2, no_proj=None):
+    s = _f64(sino)
+    out = np.empty((vol_z2, vol_x2), np.float64)
+    m, mp = _mask(no_proj)
+    lib().orc_back_proj_wbp(s, s.shape[1], proj_x, _f32(cs), mp, s.shape[0], out, vol_x2,
+                            vol_z2)
+    return out
+
+
+def wbp_crop(img2, vol_x, vol_z):
+    out = np.empty((vol_z, vol_x), np.float64)
+    lib().orc_wbp_crop(_f64(img2), vol_x, vol_z, out)
+    return out
+
+
+def tomo_sart(sino, proj_x, cs, vol, start_tilt, n_sart_tilts, iters, n_subsets,
+              relax=-1.0, order=0, positivity=1, fill_tbl=None, no_proj=None):
+    s = _f64(sino).copy()
+    v = _f64(vol).copy()
+    m, mp = _mask(no_proj)
+    if fill_tbl is None:
+        fill_tbl = np.zeros(v.size, np.int32)
+    lib().orc_tomo_sart(s, s.shape[0], s.shape[1], proj_x, _f32(cs), mp, v, v.shape[1],
+                        v.shape[0], start_tilt, n_sart_tilts, iters, n_subsets, relax, order,
+                        positivity, _i32(fill_tbl))
+    return v, s
+
+
+def tomo_wbp(sino, proj_x, cs, vol_x, vol_z, no_proj=None):
+    s = _f64(sino).copy()
+    v = np.zeros((vol_z, vol_x), np.float64)
+    m, mp = _mask(no_proj)
+    lib().orc_tomo_wbp(s, s.shape[0], s.shape[1], proj_x, _f32(cs), mp, v, vol_x, vol_z)
+    return v, s
+
+
+def corr_proj(raw, axis, shift, out_x, out_y, out_pad_x, random_fill=0, table=None):
+    raw = _f32(raw)
+    out = np.zeros((out_y, out_pad_x), np.float64)
+    tp = None
+    if table is not None:
+        table = _i32(table)
+        tp = table.ctypes.data_as(ctypes.c_void_p)
+    lib().orc_corr_proj(raw, raw.shape[1], raw.shape[0], axis, shift[0], shift[1], out,
+                        out_x, out_y, out_pad_x, random_fill, tp)
+    return out
+
+
+def prestage(raw, axes, shifts, random_fill=1, seed=0):
+    """raw: [n][raw_y][raw_x] float32 -> aligned [n][aln_y][aln_x] float64."""
+    raw = _f32(raw)
+    n, ry, rx = raw.shape
+    ax, ay = aligned_size(rx, ry, float(np.float32(axes[0])))
+    out = np.empty((n, ay, ax), np.float64)
+    sh = None
+    if shifts is not None:
+        sh = _f32(shifts)
+    lib().orc_prestage(raw, n, rx, ry, _f32(axes),
+                       None if sh is None else sh.ctypes.data_as(ctypes.c_void_p),
+                       random_fill, seed, out, ax, ay)
+    return out
+
+
+def recon_slices(aligned, tilts, method, vol_z, slices, start_tilt=0, n_sart_tilts=None,
+                 iters=20, n_subsets=5, relax=-1.0, order=0, positivity=1, seed=0,
+                 no_proj=None):
+    """Reference per-series recon (CDoWbpRecon / CDoSartRecon) on selected slices.
+
+    aligned: [n][proj_y][proj_x]; returns [len(slices)][vol_z][vol_x] in the
+    reference frame layout (z flipped)."""
+    a = _f64(aligned)
+    n, py, px = a.shape
+    if n_sart_tilts is None:
+        n_sart_tilts = n - start_tilt
+    sl = _i32(slices)
+    vol_x = px // 2 * 2
+    out = np.empty((len(sl), vol_z, vol_x), np.float64)
+    m, mp = _mask(no_proj)
+    lib().orc_recon_slices(a, n, px, py, _f32(tilts), mp, method, vol_z, start_tilt,
+                           n_sart_tilts, iters, n_subsets, relax, order, positivity, seed,
+                           sl, len(sl), out)
+    return out
+
+
+def flip_vol(vol_xzy, flip=1):
+    v = _f64(vol_xzy)
+    ny, nz, nx = v.shape
+    out = np.empty((nz, ny, nx), np.float64)
+    lib().orc_flip_vol(v, nx, nz, ny, flip, out)
+    return out
+
+
+def max_threads():
+    return int(lib().orc_max_threads())

@@ -1,0 +1,639 @@
+/*
+ * oracle.c -- FP64 CPU restatement of AreTomo3's reconstruction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the reported CPU baseline).
+ * The product never links or calls this file.
+ *
+ * Each function cites the reference file:line it restates (paths relative to
+ * the AreTomo3 source tree).  Build: see oracle/Makefile (gcc -O2 -fopenmp
+ * -ffp-contract=off).
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_EMPTY (-1e10)   /* sentinel threshold used throughout the reference */
+#define ORC_SENT  (-1e30)   /* sentinel value the reference writes */
+
+int orc_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---------------------------------------------------------------------- */
+/* Random index table for the empty-pixel fill.                            */
+/* The reference draws it with cuRAND seeded by time(0)                    */
+/* (MaUtil/GGenRandoms.cu:7-21,34-49), so its values are not reproducible. */
+/* The rebuild makes the table an explicit seeded input:                   */
+/*   r_i   = splitmix64(seed + i * 0xD1B54A32D192ED03)                      */
+/*   tbl_i = ((r_i >> 32) * n) >> 32            (uniform in [0, n))         */
+/* ---------------------------------------------------------------------- */
+uint64_t orc_splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+void orc_fill_table(uint64_t seed, int n, int* table)
+{
+    for (int i = 0; i < n; i++) {
+        uint64_t r = orc_splitmix64(seed + (uint64_t)i * 0xD1B54A32D192ED03ull);
+        table[i] = (int)(((r >> 32) * (uint64_t)n) >> 32);
+    }
+}
+
+/* Recon/CTomoBase.cpp:55-65: angle = 0.017453f * tilt (float product),
+ * cos/sin evaluated in double and stored as float. */
+void orc_cos_sin(const float* tilts_deg, int n, float* cos_sin)
+{
+    const float rad = 0.017453f;
+    for (int i = 0; i < n; i++) {
+        float ang = rad * tilts_deg[i];
+        cos_sin[2 * i] = (float)cos((double)ang);
+        cos_sin[2 * i + 1] = (float)sin((double)ang);
+    }
+}
+
+/* AreTomo/Util/CSplitItems.cpp:25-48: equal blocks, the first (n % s)
+ * blocks one larger, contiguous starts. */
+void orc_split_items(int n_items, int n_splits, int* starts, int* sizes)
+{
+    int size = n_items / n_splits;
+    for (int i = 0; i < n_splits; i++) sizes[i] = size;
+    int left = n_items % n_splits;
+    for (int i = 0; i < left; i++) sizes[i] += 1;
+    starts[0] = 0;
+    for (int i = 1; i < n_splits; i++) starts[i] = starts[i - 1] + sizes[i - 1];
+}
+
+/* Tilt indices of one SART subset.  order 0 = contiguous blocks exactly as
+ * Recon/CTomoSart.cpp:60-61,74-80 (CSplitItems over [start, start+n)).
+ * order 1 = interleaved: subset s = {start + s + k*n_subsets}. */
+int orc_subset_tilts(int start_tilt, int n_tilts, int n_subsets, int order,
+                     int subset, int* tilts)
+{
+    int n = 0;
+    if (order == 0) {
+        int* starts = (int*)malloc(sizeof(int) * n_subsets);
+        int* sizes = (int*)malloc(sizeof(int) * n_subsets);
+        orc_split_items(n_tilts, n_subsets, starts, sizes);
+        for (int k = 0; k < sizes[subset]; k++)
+            tilts[n++] = start_tilt + starts[subset] + k;
+        free(starts);
+        free(sizes);
+    } else {
+        for (int k = subset; k < n_tilts; k += n_subsets) tilts[n++] = start_tilt + k;
+    }
+    return n;
+}
+
+/* Correct/CCorrectUtil.cpp:11-21 (note the 3.14 there). */
+void orc_aligned_size(int raw_x, int raw_y, float tilt_axis_deg, int* aln_x, int* aln_y)
+{
+    double rot = fabs(sin(tilt_axis_deg * 3.14 / 180.0));
+    *aln_x = raw_x;
+    *aln_y = raw_y;
+    if (rot <= 0.707) return;
+    *aln_x = raw_y;
+    *aln_y = raw_x;
+}
+
+/* ---------------------------------------------------------------------- */
+/* FP64 mixed-radix FFT (unnormalised).  sign = -1 forward, +1 inverse.    */
+/* Used where the reference calls cuFFT (MaUtil/GFFT1D.cu:45-88,          */
+/* MaUtil/CCufft2D.cpp:23-123).  Recursive decimation in time over the     */
+/* smallest prime factor; twiddles from one double cos/sin table.          */
+/* ---------------------------------------------------------------------- */
+static int smallest_factor(int n)
+{
+    if (n % 2 == 0) return 2;
+    for (int p = 3; p * p <= n; p += 2)
+        if (n % p == 0) return p;
+    return n;
+}
+
+static void fft_rec(const double* xr, const double* xi, int s, int n,
+                    double* yr, double* yi, const double* wr, const double* wi,
+                    int big_n, double* tr, double* ti)
+{
+    if (n == 1) {
+        yr[0] = xr[0];
+        yi[0] = xi[0];
+        return;
+    }
+    int p = smallest_factor(n), m = n / p;
+    for (int r = 0; r < p; r++)
+        fft_rec(xr + (size_t)r * s, xi + (size_t)r * s, s * p, m, yr + r * m,
+                yi + r * m, wr, wi, big_n, tr, ti);
+    int step = big_n / n;
+    for (int k = 0; k < n; k++) {
+        double ar = 0.0, ai = 0.0;
+        int km = k % m;
+        for (int r = 0; r < p; r++) {
+            long e = ((long)r * k) % n;
+            double cr = wr[e * step], ci = wi[e * step];
+            double vr = yr[r * m + km], vi = yi[r * m + km];
+            ar += vr * cr - vi * ci;
+            ai += vr * ci + vi * cr;
+        }
+        tr[k] = ar;
+        ti[k] = ai;
+    }
+    memcpy(yr, tr, sizeof(double) * n);
+    memcpy(yi, ti, sizeof(double) * n);
+}
+
+void orc_fft(double* re, double* im, int n, int sign)
+{
+    double* buf = (double*)malloc(sizeof(double) * 6 * (size_t)n);
+    double *wr = buf, *wi = buf + n, *yr = buf + 2 * n, *yi = buf + 3 * n;
+    double *tr = buf + 4 * n, *ti = buf + 5 * n;
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int e = 0; e < n; e++) {
+        double a = two_pi * (double)e / (double)n;
+        wr[e] = cos(a);
+        wi[e] = sign * sin(a);
+    }
+    fft_rec(re, im, 1, n, yr, yi, wr, wi, n, tr, ti);
+    memcpy(re, yr, sizeof(double) * n);
+    memcpy(im, yi, sizeof(double) * n);
+    free(buf);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Weighting                                                               */
+/* ---------------------------------------------------------------------- */
+
+/* Recon/GWeightProjs.cu:7-19 (launch :41-47): s[t][x] *= cos_t / VolZ for
+ * x < proj_x. */
+void orc_weight_projs(double* sino, int n_tilts, int pad_x, int proj_x,
+                      const float* cos_sin, int vol_z)
+{
+    for (int t = 0; t < n_tilts; t++) {
+        double w = (double)cos_sin[2 * t] / (double)vol_z;
+        for (int x = 0; x < proj_x; x++) sino[(size_t)t * pad_x + x] *= w;
+    }
+}
+
+/* Recon/GRWeight.cu:7-23,57-69 with MaUtil/GFFT1D.cu:45-88: row R2C of
+ * length fft_size, bin x in [0, N/2] scaled by
+ *   (1/N) * 2 r (0.55 + 0.45 cos(6.2831852 r)),  r = x / N,
+ * then unnormalised C2R.  The filter is real and even, so the complex FFT of
+ * the real row followed by the symmetric scaling and a complex inverse gives
+ * the same Hermitian C2R result. */
+void orc_rweight(double* sino, int n_tilts, int pad_x, int fft_size)
+{
+    int n = fft_size;
+    const double c2pi = (double)6.2831852f;
+    double* w = (double*)malloc(sizeof(double) * (n / 2 + 1));
+    for (int x = 0; x <= n / 2; x++) {
+        double r = (double)x / (double)n;
+        w[x] = 2.0 * r * (0.55 + 0.45 * cos(c2pi * r)) / (double)n;
+    }
+    double* re = (double*)malloc(sizeof(double) * n);
+    double* im = (double*)malloc(sizeof(double) * n);
+    for (int t = 0; t < n_tilts; t++) {
+        double* row = sino + (size_t)t * pad_x;
+        for (int x = 0; x < n; x++) {
+            re[x] = row[x];
+            im[x] = 0.0;
+        }
+        orc_fft(re, im, n, -1);
+        for (int k = 0; k < n; k++) {
+            int kk = k <= n / 2 ? k : n - k;
+            re[k] *= w[kk];
+            im[k] *= w[kk];
+        }
+        orc_fft(re, im, n, +1);
+        for (int x = 0; x < n; x++) row[x] = re[x];
+    }
+    free(w);
+    free(re);
+    free(im);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Back-projection, forward projection, difference                         */
+/* ---------------------------------------------------------------------- */
+
+/* Recon/GBackProj.cu:12-60.  For each voxel: linear interpolation of every
+ * listed, not-excluded tilt whose coordinate u = fx cos + fz sin + ProjX/2
+ * lies in [0, ProjX-2]; samples <= -1e10 skipped; v += relax * sum / count;
+ * positivity clamp (bSart); <= -1e10 -> -1e30; count 0 leaves v unchanged. */
+void orc_back_proj(const double* sino, int pad_x, int proj_x,
+                   const float* cos_sin, const unsigned char* no_proj,
+                   const int* tilts, int n_list, int positivity, double relax,
+                   double* vol, int vol_x, int vol_z)
+{
+    double cx = proj_x / 2.0;
+    int end_x = proj_x - 2;
+    for (int z = 0; z < vol_z; z++) {
+        double fz = z - vol_z * 0.5;
+        for (int x = 0; x < vol_x; x++) {
+            double fx = x - vol_x * 0.5;
+            double sum = 0.0;
+            int count = 0;
+            for (int k = 0; k < n_list; k++) {
+                int t = tilts[k];
+                if (no_proj != 0 && no_proj[t]) continue;
+                double c = cos_sin[2 * t], s = cos_sin[2 * t + 1];
+                double u = fx * c + fz * s + cx;
+                if (u < 0 || u > end_x) continue;
+                int iu = (int)u;
+                double f = u - iu;
+                const double* row = sino + (size_t)t * pad_x;
+                double v = row[iu] * (1 - f) + row[iu + 1] * f;
+                if (v <= ORC_EMPTY) continue;
+                sum += v;
+                count += 1;
+            }
+            if (count <= 0) continue;
+            size_t i = (size_t)z * vol_x + x;
+            double val = relax * sum / count + vol[i];
+            if (val > ORC_EMPTY) vol[i] = positivity ? (val > 0 ? val : 0.0) : val;
+            else vol[i] = ORC_SENT;
+        }
+    }
+}
+
+/* Recon/GForProj.cu:10-76: ray (tilt t, detector x) of L = (int)(VolZ/cos +
+ * 1.5) unit steps centred on the slice; sample k at
+ *   z = VolZ/2 + (k - L/2) cos,  x = (xd - ProjX/2)/cos + VolX/2 - (k - L/2) sin;
+ * bilinear inside [0,VolX-2]x[0,VolZ-2]; samples < -1e10 skipped; mean of the
+ * rest, -1e30 when none.  Output row t of fwd (pad_x stride). */
+void orc_for_proj(const double* vol, int vol_x, int vol_z,
+                  const float* cos_sin, const int* tilts, int n_list,
+                  int proj_x, int pad_x, double* fwd)
+{
+    int end_x = vol_x - 2, end_z = vol_z - 2;
+    for (int k = 0; k < n_list; k++) {
+        int t = tilts[k];
+        double c = cos_sin[2 * t], s = cos_sin[2 * t + 1];
+        int len = (int)(vol_z / c + 1.5);
+        for (int xd = 0; xd < proj_x; xd++) {
+            double xp = xd - 0.5 * proj_x;
+            double tx = xp / c + vol_x * 0.5;
+            double tz = vol_z * 0.5;
+            double sum = 0.0, count = 0.0;
+            for (int i = 0; i < len; i++) {
+                double fzk = i - len * 0.5;
+                double x = tx - fzk * s;
+                double z = tz + fzk * c;
+                if (x < 0 || z < 0 || x > end_x || z > end_z) continue;
+                int ix = (int)x, iz = (int)z;
+                double ax = x - ix, az = z - iz;
+                const double* r0 = vol + (size_t)iz * vol_x + ix;
+                const double* r1 = r0 + vol_x;
+                double v = r0[0] * (1 - ax) * (1 - az) + r0[1] * ax * (1 - az)
+                         + r1[0] * (1 - ax) * az + r1[1] * ax * az;
+                if (v < ORC_EMPTY) continue;
+                sum += v;
+                count += 1.0;
+            }
+            fwd[(size_t)t * pad_x + xd] = count <= 0 ? ORC_SENT : sum / count;
+        }
+    }
+}
+
+/* Recon/GDiffProj.cu:7-21 (used in place, CTomoSart.cpp:103-113):
+ * d = raw - fwd, or 0 if either is < -1e10. */
+void orc_diff_proj(const double* raw, double* fwd, const int* tilts,
+                   int n_list, int proj_x, int pad_x)
+{
+    for (int k = 0; k < n_list; k++) {
+        size_t base = (size_t)tilts[k] * pad_x;
+        for (int x = 0; x < proj_x; x++) {
+            size_t i = base + x;
+            if (fwd[i] < ORC_EMPTY) fwd[i] = 0.0;
+            else if (raw[i] < ORC_EMPTY) fwd[i] = 0.0;
+            else fwd[i] = raw[i] - fwd[i];
+        }
+    }
+}
+
+/* MaUtil/GFillEmpty2D.cu:7-143: pixels < -1e10 are empty; each empty pixel
+ * follows the random index chain (<= 100 hops) to the first non-empty pixel
+ * and takes its value, else the mean of all non-empty pixels (0 if none). */
+void orc_fill_empty(double* img, int w, int h, int pad_x, const int* table)
+{
+    int n = w * h;
+    unsigned char* empty = (unsigned char*)malloc(n);
+    double sum = 0.0, count = 0.0;
+    for (int i = 0; i < n; i++) {
+        double v = img[(size_t)(i / w) * pad_x + i % w];
+        empty[i] = v < ORC_EMPTY;
+        if (!empty[i]) {
+            sum += v;
+            count += 1.0;
+        }
+    }
+    double mean = count <= 0 ? 0.0 : sum / count;
+    for (int i = 0; i < n; i++) {
+        if (!empty[i]) continue;
+        double val = mean;
+        int next = table[i];
+        for (int j = 0; j < 100; j++) {
+            if (empty[next]) {
+                next = table[next];
+                continue;
+            }
+            val = img[(size_t)(next / w) * pad_x + next % w];
+            break;
+        }
+        img[(size_t)(i / w) * pad_x + i % w] = val;
+    }
+    free(empty);
+}
+
+/* ---------------------------------------------------------------------- */
+/* SART (Recon/CTomoSart.cpp:54-125)                                        */
+/* ---------------------------------------------------------------------- */
+void orc_tomo_sart(double* sino, int n_tilts, int pad_x, int proj_x,
+                   const float* cos_sin, const unsigned char* no_proj,
+                   double* vol, int vol_x, int vol_z, int start_tilt,
+                   int n_sart_tilts, int iters, int n_subsets, double relax,
+                   int order, int positivity, const int* vol_fill_table)
+{
+    /* weighting of all projections (:65-66) */
+    orc_weight_projs(sino, n_tilts, pad_x, proj_x, cos_sin, vol_z);
+    /* initial back-projection of every projection, relax 1 (:68-69) */
+    int* all = (int*)malloc(sizeof(int) * n_tilts);
+    for (int t = 0; t < n_tilts; t++) all[t] = t;
+    orc_back_proj(sino, pad_x, proj_x, cos_sin, no_proj, all, n_tilts,
+                  positivity, 1.0, vol, vol_x, vol_z);
+    free(all);
+    /* relax = 1/nSubsets in the reference (:71) */
+    double r = relax > 0 ? relax : 1.0 / n_subsets;
+    double* fwd = (double*)calloc((size_t)n_tilts * pad_x, sizeof(double));
+    int* list = (int*)malloc(sizeof(int) * (n_sart_tilts + 1));
+    for (int it = 0; it < iters; it++) {
+        for (int s = 0; s < n_subsets; s++) {
+            int n = orc_subset_tilts(start_tilt, n_sart_tilts, n_subsets, order, s, list);
+            orc_for_proj(vol, vol_x, vol_z, cos_sin, list, n, proj_x, pad_x, fwd);
+            orc_diff_proj(sino, fwd, list, n, proj_x, pad_x);
+            orc_back_proj(fwd, pad_x, proj_x, cos_sin, no_proj, list, n,
+                          positivity, r, vol, vol_x, vol_z);
+        }
+    }
+    free(list);
+    free(fwd);
+    /* GFillEmpty2D on the slice (:86-89) */
+    orc_fill_empty(vol, vol_x, vol_z, vol_x, vol_fill_table);
+}
+
+/* ---------------------------------------------------------------------- */
+/* WBP (Recon/CTomoWbp.cpp:38-60, Recon/GBackProjWbp.cu:14-143)            */
+/* ---------------------------------------------------------------------- */
+
+/* Recon/GBackProjWbp.cu:14-53: back-projection onto the 2x grid
+ * (vol_x2 = 2 VolX, vol_z2 = 2 VolZ):
+ *   u = (fx cos + fz sin)/2 + (ProjX/2) + 0.5,  fx = x - vol_x2/2, fz = z - vol_z2/2;
+ * mean over valid tilts, 0 when none. */
+void orc_back_proj_wbp(const double* sino, int pad_x, int proj_x,
+                       const float* cos_sin, const unsigned char* no_proj,
+                       int n_tilts, double* img2, int vol_x2, int vol_z2)
+{
+    int cx = proj_x / 2;
+    int end_x = proj_x - 2;
+    for (int z = 0; z < vol_z2; z++) {
+        double fz = z - vol_z2 / 2.0;
+        for (int x = 0; x < vol_x2; x++) {
+            double fx = x - vol_x2 / 2.0;
+            double sum = 0.0;
+            int count = 0;
+            for (int t = 0; t < n_tilts; t++) {
+                if (no_proj != 0 && no_proj[t]) continue;
+                double c = cos_sin[2 * t], s = cos_sin[2 * t + 1];
+                double u = (fx * c + fz * s) / 2 + cx + 0.5;
+                if (u < 0 || u > end_x) continue;
+                int iu = (int)u;
+                double f = u - iu;
+                const double* row = sino + (size_t)t * pad_x;
+                double v = row[iu] * (1 - f) + row[iu + 1] * f;
+                if (v <= ORC_EMPTY) continue;
+                sum += v;
+                count += 1;
+            }
+            img2[(size_t)z * vol_x2 + x] = count <= 0 ? 0.0 : sum / count;
+        }
+    }
+}
+
+/* Recon/GBackProjWbp.cu:126-141 with MaUtil/CCufft2D.cpp:65-80 (forward R2C,
+ * x 1/(2VolX*2VolZ)), MaUtil/GFourierResize2D.cu:7-39 (crop: output row
+ * j <= VolZ/2 from input row j, j > VolZ/2 from input row j + VolZ; columns
+ * 0..VolX/2) and MaUtil/CCufft2D.cpp:108-123 (unnormalised C2R VolZ x VolX).
+ * The crop leaves the Nyquist row/column non-Hermitian; the C2R is defined
+ * here as a complex inverse along z followed by a real inverse along x that
+ * ignores the imaginary parts of the DC and Nyquist bins (numpy irfft2). */
+void orc_wbp_crop(const double* img2, int vol_x, int vol_z, double* vol)
+{
+    int nx2 = 2 * vol_x, nz2 = 2 * vol_z, kx_n = vol_x / 2 + 1;
+    double* ar = (double*)malloc(sizeof(double) * (size_t)nz2 * kx_n);
+    double* ai = (double*)malloc(sizeof(double) * (size_t)nz2 * kx_n);
+    int big = nx2 > nz2 ? nx2 : nz2;
+    double* re = (double*)malloc(sizeof(double) * big);
+    double* im = (double*)malloc(sizeof(double) * big);
+    /* forward along x (rows of the 2x grid) */
+    for (int z = 0; z < nz2; z++) {
+        for (int x = 0; x < nx2; x++) {
+            re[x] = img2[(size_t)z * nx2 + x];
+            im[x] = 0.0;
+        }
+        orc_fft(re, im, nx2, -1);
+        for (int k = 0; k < kx_n; k++) {
+            ar[(size_t)z * kx_n + k] = re[k];
+            ai[(size_t)z * kx_n + k] = im[k];
+        }
+    }
+    double norm = 1.0 / ((double)nx2 * (double)nz2);
+    double* hr = (double*)malloc(sizeof(double) * (size_t)vol_z * kx_n);
+    double* hi = (double*)malloc(sizeof(double) * (size_t)vol_z * kx_n);
+    /* forward along z, normalise, crop, inverse along z */
+    for (int k = 0; k < kx_n; k++) {
+        for (int z = 0; z < nz2; z++) {
+            re[z] = ar[(size_t)z * kx_n + k];
+            im[z] = ai[(size_t)z * kx_n + k];
+        }
+        orc_fft(re, im, nz2, -1);
+        double* gr = (double*)malloc(sizeof(double) * vol_z);
+        double* gi = (double*)malloc(sizeof(double) * vol_z);
+        for (int j = 0; j < vol_z; j++) {
+            int src = j <= vol_z / 2 ? j : j + vol_z;
+            gr[j] = re[src] * norm;
+            gi[j] = im[src] * norm;
+        }
+        orc_fft(gr, gi, vol_z, +1);
+        for (int z = 0; z < vol_z; z++) {
+            hr[(size_t)z * kx_n + k] = gr[z];
+            hi[(size_t)z * kx_n + k] = gi[z];
+        }
+        free(gr);
+        free(gi);
+    }
+    /* real inverse along x */
+    for (int z = 0; z < vol_z; z++) {
+        const double* r = hr + (size_t)z * kx_n;
+        const double* i = hi + (size_t)z * kx_n;
+        re[0] = r[0];
+        im[0] = 0.0;
+        re[vol_x / 2] = r[vol_x / 2];
+        im[vol_x / 2] = 0.0;
+        for (int k = 1; k < vol_x / 2; k++) {
+            re[k] = r[k];
+            im[k] = i[k];
+            re[vol_x - k] = r[k];
+            im[vol_x - k] = -i[k];
+        }
+        orc_fft(re, im, vol_x, +1);
+        for (int x = 0; x < vol_x; x++) vol[(size_t)z * vol_x + x] = re[x];
+    }
+    free(ar);
+    free(ai);
+    free(hr);
+    free(hi);
+    free(re);
+    free(im);
+}
+
+void orc_tomo_wbp(double* sino, int n_tilts, int pad_x, int proj_x,
+                  const float* cos_sin, const unsigned char* no_proj,
+                  double* vol, int vol_x, int vol_z)
+{
+    orc_weight_projs(sino, n_tilts, pad_x, proj_x, cos_sin, vol_z);
+    orc_rweight(sino, n_tilts, pad_x, proj_x);
+    int nx2 = 2 * vol_x, nz2 = 2 * vol_z;
+    double* img2 = (double*)malloc(sizeof(double) * (size_t)nx2 * nz2);
+    orc_back_proj_wbp(sino, pad_x, proj_x, cos_sin, no_proj, n_tilts, img2, nx2, nz2);
+    orc_wbp_crop(img2, vol_x, vol_z, vol);
+    free(img2);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Pre-stage: Correct/GCorrPatchShift.cu:58-100 (global shift + tilt-axis   */
+/* rotation, nearest neighbour) and :134-154 (random fill)                 */
+/* ---------------------------------------------------------------------- */
+void orc_corr_proj(const float* raw, int raw_x, int raw_y, float tilt_axis_deg,
+                   float shift_x, float shift_y, double* out, int out_x,
+                   int out_y, int out_pad_x, int random_fill,
+                   const int* fill_table)
+{
+    const float d2r = 3.141592654f / 180.0f;
+    float ang = tilt_axis_deg * d2r;
+    double ca = cos((double)ang), sa = sin((double)ang);
+    for (int y = 0; y < out_y; y++) {
+        for (int x = 0; x < out_x; x++) {
+            double xc = x + 0.5 - out_x * 0.5;
+            double yc = y + 0.5 - out_y * 0.5;
+            double xr = xc * ca - yc * sa;
+            double yr = xc * sa + yc * ca;
+            xr += (shift_x + raw_x * 0.5);
+            yr += (shift_y + raw_y * 0.5);
+            int ix = (int)(xr + 0.5);
+            int iy = (int)(yr + 0.5);
+            double v = ORC_SENT;
+            if (ix >= 0 && ix < raw_x && iy >= 0 && iy < raw_y)
+                v = raw[(size_t)iy * raw_x + ix];
+            out[(size_t)y * out_pad_x + x] = v;
+        }
+    }
+    if (random_fill) orc_fill_empty(out, out_x, out_y, out_pad_x, fill_table);
+}
+
+/* Correct/CCorrTomoStack.cpp:185-222 per tilt, sizes from
+ * CCorrectUtil::CalcAlignedSize with the axis of frame 0
+ * (AreTomo/CAreTomoMain.cpp:612-615).  Output [t][aln_y][aln_x]. */
+void orc_prestage(const float* raw, int n_tilts, int raw_x, int raw_y,
+                  const float* tilt_axis_deg, const float* shifts_xy,
+                  int random_fill, uint64_t fill_seed, double* aligned,
+                  int aln_x, int aln_y)
+{
+    int pad = (aln_x / 2 + 1) * 2;
+    int* table = (int*)malloc(sizeof(int) * (size_t)aln_x * aln_y);
+    orc_fill_table(fill_seed, aln_x * aln_y, table);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < n_tilts; t++) {
+        double* buf = (double*)malloc(sizeof(double) * (size_t)pad * aln_y);
+        float sx = shifts_xy ? shifts_xy[2 * t] : 0.0f;
+        float sy = shifts_xy ? shifts_xy[2 * t + 1] : 0.0f;
+        orc_corr_proj(raw + (size_t)t * raw_x * raw_y, raw_x, raw_y,
+                      tilt_axis_deg[t], sx, sy, buf, aln_x, aln_y, pad,
+                      random_fill, table);
+        for (int y = 0; y < aln_y; y++)
+            memcpy(aligned + ((size_t)t * aln_y + y) * aln_x, buf + (size_t)y * pad,
+                   sizeof(double) * aln_x);
+        free(buf);
+    }
+    free(table);
+}
+
+/* Per-series slice loop: Recon/CDoWbpRecon.cpp:49-133 and
+ * Recon/CDoSartRecon.cpp:193-279.  VolX = ProjX/2*2, pad = (ProjX/2+1)*2;
+ * the sinogram of slice y is row y of every tilt; each slice starts from a
+ * zeroed volume; the result is stored z-flipped (frame y, row VolZ-1-z).
+ * Computes only the listed slices: out[k] = frame slices[k]. */
+void orc_recon_slices(const double* aligned, int n_tilts, int proj_x,
+                      int proj_y, const float* tilts_deg,
+                      const unsigned char* no_proj, int method, int vol_z,
+                      int start_tilt, int n_sart_tilts, int iters,
+                      int n_subsets, double relax, int order, int positivity,
+                      uint64_t fill_seed, const int* slices, int n_slices,
+                      double* out)
+{
+    int vol_x = proj_x / 2 * 2;
+    int pad = (proj_x / 2 + 1) * 2;
+    int proj_eff = (pad / 2 - 1) * 2;  /* GBackProj.cu:74, GRWeight.cu:66 */
+    float* cs = (float*)malloc(sizeof(float) * 2 * n_tilts);
+    orc_cos_sin(tilts_deg, n_tilts, cs);
+    int* table = (int*)malloc(sizeof(int) * (size_t)vol_x * vol_z);
+    orc_fill_table(fill_seed ^ 0x5A5A5A5A5A5A5A5Aull, vol_x * vol_z, table);
+    (void)proj_y;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < n_slices; k++) {
+        int y = slices[k];
+        double* sino = (double*)calloc((size_t)pad * n_tilts, sizeof(double));
+        double* vol = (double*)calloc((size_t)vol_x * vol_z, sizeof(double));
+        for (int t = 0; t < n_tilts; t++)
+            memcpy(sino + (size_t)t * pad,
+                   aligned + ((size_t)t * proj_y + y) * proj_x,
+                   sizeof(double) * proj_x);
+        if (method == 0)
+            orc_tomo_wbp(sino, n_tilts, pad, proj_eff, cs, no_proj, vol, vol_x, vol_z);
+        else
+            orc_tomo_sart(sino, n_tilts, pad, proj_eff, cs, no_proj, vol, vol_x,
+                          vol_z, start_tilt, n_sart_tilts, iters, n_subsets,
+                          relax, order, positivity, table);
+        double* dst = out + (size_t)k * vol_x * vol_z;
+        for (int z = 0; z < vol_z; z++)
+            memcpy(dst + (size_t)(vol_z - 1 - z) * vol_x, vol + (size_t)z * vol_x,
+                   sizeof(double) * vol_x);
+        free(sino);
+        free(vol);
+    }
+    free(cs);
+    free(table);
+}
+
+/* DataUtil/CTiltSeries.cpp:286-305: XZY frames (frame y holds vol_z rows)
+ * -> XYZ frames; new frame index = flip ? (vol_z-1-row) : row. */
+void orc_flip_vol(const double* vol_xzy, int vol_x, int vol_z, int vol_y,
+                  int flip, double* vol_xyz)
+{
+    for (int y = 0; y < vol_z; y++) {
+        int nf = flip ? (vol_z - 1 - y) : y;
+        for (int z = 0; z < vol_y; z++)
+            memcpy(vol_xyz + ((size_t)nf * vol_y + z) * vol_x,
+                   vol_xzy + ((size_t)z * vol_z + y) * vol_x,
+                   sizeof(double) * vol_x);
+    }
+}
