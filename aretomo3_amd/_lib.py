@@ -28,7 +28,7 @@ EXPORTS = [
     "atr_plan_set_profiling", "atr_plan_kernel_stats", "atr_plan_reset_stats",
     "atr_plan_destroy", "atr_last_error_string", "atr_version", "atr_phantom_tilt_series",
     "atr_device_count", "atr_set_device", "atr_malloc", "atr_free", "atr_memcpy_h2d",
-    "atr_memcpy_d2h", "atr_memset", "atr_synchronize", "atr_stream_create",
+    "atr_memcpy_d2h", "atr_memset", "atr_copy_kernel", "atr_synchronize", "atr_stream_create",
     "atr_stream_destroy", "atr_stream_synchronize", "atr_mem_info", "atr_event_create",
     "atr_event_destroy", "atr_event_record", "atr_event_elapsed_ms",
 ]
@@ -105,6 +105,7 @@ def lib():
         "atr_memcpy_h2d": (i, [vp, vp, sz]),
         "atr_memcpy_d2h": (i, [vp, vp, sz]),
         "atr_memset": (i, [vp, i, sz]),
+        "atr_copy_kernel": (i, [vp, vp, sz, vp]),
         "atr_synchronize": (i, []),
         "atr_stream_create": (i, [ctypes.POINTER(vp)]),
         "atr_stream_destroy": (i, [vp]),

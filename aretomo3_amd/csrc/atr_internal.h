@@ -21,11 +21,15 @@ constexpr float kSentinel = -1e30f;
 // lane = x, each thread walks BP_TZ rows of z.
 constexpr int BP_TX = 256;
 constexpr int BP_TZ = 16;
-constexpr int BP_CHUNK = 8;                  // tilts staged in LDS at once
+constexpr int BP_CHUNK = 16;                 // tilts staged in LDS at once
 constexpr int BP_WIN = BP_TX + BP_TZ + 16;   // staged pairs per tilt
-// Forward projection: one ray per lane, FP_RAYS rays per workgroup.
-constexpr int FP_RAYS = 256;
+// Forward projection: a workgroup owns one slice and a group of tilts, one
+// wave per (tilt, part of its rays), fp_rpl rays per lane; the slice is
+// streamed through LDS in z-bands of bilinear cells, sized so that two
+// workgroups share a CU.
 constexpr int FP_WAVE = 64;
+constexpr int FP_MAX_WAVES = 16;
+constexpr int FP_LDS_BUDGET = 150 * 1024;
 
 enum Kernel {
     K_BP = 0,
@@ -53,10 +57,14 @@ struct Geometry {
     std::vector<uint8_t> bp_class;     // [N][ztiles][xtiles] 0 empty, 1 full, 2 partial
     std::vector<int2> wbp_lohi;        // [N][2VolZ] x2-interval of GBackProjWbp inclusion
     std::vector<int2> fp_range;        // [N][P] valid sample interval of each ray
-    std::vector<int4> fp_tile;         // [N][P/64] out_lo, core_lo, core_hi, out_hi
+    std::vector<int2> fp_span;         // [N] first/last ray that meets the slice
+    std::vector<uint8_t> fp_full;      // [N][wpt][nbands] band window used by every ray
+    int fp_rpl = 16, fp_wpt = 1;       // rays per lane, waves per tilt
     std::vector<int> fp_len;           // [N] ray length L
     std::vector<int> fp_geo_off;       // [N] offset into fp_geo
-    std::vector<float4> fp_geo;        // [sum L] {row offset bits, wz, q, 0}
+    std::vector<float4> fp_geo;        // [sum L] {row index bits, wz, q, 0}
+    std::vector<int2> fp_band;         // [N][nbands] sample range whose rows lie in a z-band
+    int fp_bz = 0, fp_nbands = 0;      // z rows per band
 };
 
 struct Plan {
@@ -89,10 +97,12 @@ struct Plan {
     uint8_t* d_bp_class = nullptr;
     int2* d_wbp_lohi = nullptr;
     int2* d_fp_range = nullptr;
-    int4* d_fp_tile = nullptr;
+    int2* d_fp_span = nullptr;
+    uint8_t* d_fp_full = nullptr;
     int* d_fp_geo_off = nullptr;
     int* d_fp_len = nullptr;
     float4* d_fp_geo = nullptr;
+    int2* d_fp_band = nullptr;
     TiltRow* d_lists = nullptr;       // all_tilts then every subset's BP list
     int* d_fp_lists = nullptr;        // every subset's FP list
     int* d_fill_aln = nullptr;        // [aln_x*aln_y]
@@ -150,7 +160,8 @@ void build_bp_classes(const std::vector<int2>& lohi, int n, int vol_x, int vol_z
 void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                          std::vector<int2>& lohi);
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
-                     Geometry& g);
+                     int bz, Geometry& g);
+int fp_band_rows(int vol_x);
 void split_items(int n_items, int n_splits, std::vector<int>& starts, std::vector<int>& sizes);
 void fill_table(uint64_t seed, int n, std::vector<int>& table);
 void aligned_size(int raw_x, int raw_y, float axis_deg, int* ax, int* ay);
@@ -176,5 +187,6 @@ void launch_wbp_bp_rowfft(const Plan& p, int y0, int ny, hipStream_t s);
 void launch_wbp_colfft(const Plan& p, int y0, int ny, hipStream_t s);
 void launch_wbp_rowc2r(const Plan& p, int y0, int ny, float* out, hipStream_t s);
 void launch_load_volume(const Plan& p, const float* in, hipStream_t s);
+void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 }  // namespace atr

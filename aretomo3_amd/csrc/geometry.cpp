@@ -172,11 +172,12 @@ void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol
 // Recon/GForProj.cu:131-149: ray (t, xd) has L = (int)(VolZ/cos + 1.5)
 // samples; sample k at fzk = k - L*0.5, x = ((xd - 0.5 ProjX)/cos + VolX*0.5)
 // - fzk sin, z = VolZ*0.5 + fzk cos, used iff inside [0,VolX-2]x[0,VolZ-2].
-// Tables: per ray its sample interval, per 64-ray tile the union and the
-// common core of those intervals, per (tilt, k) the row offset, the z weight
-// and fzk*sin.
+// Tables: per ray its sample interval; per tilt the run of rays that meet
+// the slice; per (tilt, k) the cell row, the z weight and fzk*sin; per
+// (tilt, band) the samples whose cell row lies in the band; per (tilt, wave
+// part, band) whether every ray of the part uses the whole band window.
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
-                     Geometry& g)
+                     int bz, Geometry& g)
 {
     const int end_x = vol_x - 2, end_z = vol_z - 2;
     const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
@@ -195,7 +196,11 @@ bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, 
     }
     g.fp_geo.assign(total, make_float4(0, 0, 0, 0));
     g.fp_range.assign((size_t)n * proj, make_int2(0, -1));
-    g.fp_tile.assign((size_t)n * ntile, make_int4(0, 0, -1, -1));
+    g.fp_span.assign(n, make_int2(0, -1));
+    const int nb = (vol_z - 1 + bz - 1) / bz;
+    g.fp_bz = bz;
+    g.fp_nbands = nb;
+    g.fp_band.assign((size_t)n * nb, make_int2(0, -1));
     parallel_tilts(n, [&](int t) {
         double c = cs[2 * t], s = cs[2 * t + 1];
         int len = g.fp_len[t];
@@ -214,7 +219,12 @@ bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, 
             int iz = (int)zc;
             float wz = (float)(zc - iz);
             float q = (float)(fzk * s);
-            geo[k] = make_float4(__builtin_bit_cast(float, iz * vol_x), wz, q, 0.0f);
+            geo[k] = make_float4(__builtin_bit_cast(float, iz), wz, q, 0.0f);
+            if (zin) {
+                int2& bb = g.fp_band[(size_t)t * nb + iz / bz];
+                if (bb.y < bb.x) bb = make_int2(k, k);
+                else bb.y = k;
+            }
         }
         for (int xd = 0; xd < proj; xd++) {
             double xp = xd - 0.5 * proj;
@@ -234,27 +244,44 @@ bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, 
             }
             g.fp_range[(size_t)t * proj + xd] = lo < 0 ? make_int2(0, -1) : make_int2(lo, hi);
         }
-        for (int w = 0; w < ntile; w++) {
-            int out_lo = 1 << 30, out_hi = -1, core_lo = -1, core_hi = 1 << 30;
-            bool all = true;
-            for (int xd = w * FP_WAVE; xd < std::min((w + 1) * FP_WAVE, proj); xd++) {
-                int2 r = g.fp_range[(size_t)t * proj + xd];
-                if (r.y < r.x) {
-                    all = false;
-                    continue;
-                }
-                out_lo = std::min(out_lo, r.x);
-                out_hi = std::max(out_hi, r.y);
-                core_lo = std::max(core_lo, r.x);
-                core_hi = std::min(core_hi, r.y);
+        // rays that meet the slice form one contiguous run (convex region)
+        int x0 = proj, x1 = -1;
+        for (int xd = 0; xd < proj; xd++) {
+            int2 r = g.fp_range[(size_t)t * proj + xd];
+            if (r.x <= r.y) {
+                x0 = std::min(x0, xd);
+                x1 = std::max(x1, xd);
             }
-            int4 tl;
-            if (out_hi < 0) tl = make_int4(0, 0, -1, -1);
-            else if (!all || core_lo > core_hi) tl = make_int4(out_lo, out_hi + 1, out_hi, out_hi);
-            else tl = make_int4(out_lo, core_lo, core_hi, out_hi);
-            g.fp_tile[(size_t)t * ntile + w] = tl;
+        }
+        g.fp_span[t] = x0 <= x1 ? make_int2(x0, x1) : make_int2(0, -1);
+    });
+    // rays per lane / waves per tilt from the widest run
+    int span_max = 1;
+    for (int t = 0; t < n; t++) span_max = std::max(span_max, g.fp_span[t].y - g.fp_span[t].x + 1);
+    g.fp_rpl = span_max <= 4 * FP_WAVE ? 4 : (span_max <= 8 * FP_WAVE ? 8 : 16);
+    g.fp_wpt = (span_max + FP_WAVE * g.fp_rpl - 1) / (FP_WAVE * g.fp_rpl);
+    const int rays_per_wave = FP_WAVE * g.fp_rpl;
+    // full[t][part][band]: every ray the wave owns uses every sample of the
+    // band's window, so the wave runs the band without per-sample tests
+    g.fp_full.assign((size_t)n * g.fp_wpt * nb, 0);
+    parallel_tilts(n, [&](int t) {
+        const int2 sp = g.fp_span[t];
+        for (int part = 0; part < g.fp_wpt; part++) {
+            const int a = sp.x + part * rays_per_wave;
+            const int e = std::min(sp.y, a + rays_per_wave - 1);
+            for (int b = 0; b < nb; b++) {
+                const int2 kb = g.fp_band[(size_t)t * nb + b];
+                bool full = true;
+                if (kb.x <= kb.y)
+                    for (int xd = a; xd <= e && full; xd++) {
+                        int2 r = g.fp_range[(size_t)t * proj + xd];
+                        full = r.x <= kb.x && r.y >= kb.y;
+                    }
+                g.fp_full[((size_t)t * g.fp_wpt + part) * nb + b] = full ? 1 : 0;
+            }
         }
     });
+    (void)ntile;
     return true;
 }
 
@@ -297,4 +324,14 @@ std::vector<float> ramp_filter(int n)
     return w;
 }
 
+}  // namespace atr
+
+namespace atr {
+// Cell rows per forward-projection band: bz rows of (VolX + 1) float4 cells
+// must fit the LDS budget of one workgroup.
+int fp_band_rows(int vol_x)
+{
+    int rows = FP_LDS_BUDGET / ((vol_x + 1) * (int)sizeof(float4));
+    return std::min(32, std::max(1, rows));
+}
 }  // namespace atr

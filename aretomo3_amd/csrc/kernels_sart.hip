@@ -176,7 +176,8 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
     int vol_x, int vol_z, int* __restrict__ vflag)
 {
     __shared__ float2 pairs[BP_CHUNK][BP_WIN];
-    __shared__ int win0[BP_CHUNK];
+    __shared__ int s_w0[BP_CHUNK], s_row[BP_CHUNK], s_tilt[BP_CHUNK], s_cls[BP_CHUNK];
+    __shared__ float2 s_cs[BP_CHUNK];
     const int tid = threadIdx.x;
     const int xt = blockIdx.x, zt = blockIdx.y, y = blockIdx.z;
     const int x0 = xt * BP_TX, z0 = zt * BP_TZ;
@@ -202,44 +203,46 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
 
     for (int c0 = 0; c0 < n_list; c0 += BP_CHUNK) {
         const int nc = min(BP_CHUNK, n_list - c0);
-        // ---- stage the row segments of this chunk ----
-        for (int j = 0; j < nc; j++) {
-            const TiltRow tr = list[c0 + j];
-            const uint8_t cl = cls[((size_t)tr.tilt * n_ztiles + zt) * n_xtiles + xt];
-            if (cl == 0) {
-                if (tid == 0) win0[j] = INT_MIN;
-                continue;
+        // ---- per-tilt window of this tile (one thread per tilt) ----
+        if (tid < nc) {
+            const TiltRow tr = list[c0 + tid];
+            const int cl = cls[((size_t)tr.tilt * n_ztiles + zt) * n_xtiles + xt];
+            const float2 sc = cs[tr.tilt];
+            const float umin = 0.5f * proj + fminf(sc.x * fxt0, sc.x * fxt1) +
+                               fminf(sc.y * (float)fz0, sc.y * fzt1);
+            s_w0[tid] = (int)floorf(umin) - 3;
+            s_row[tid] = tr.row;
+            s_tilt[tid] = tr.tilt;
+            s_cls[tid] = cl;
+            s_cs[tid] = sc;
+        }
+        __syncthreads();
+        // ---- stage the row segments: independent loads, one pass ----
+        for (int idx = tid; idx < nc * BP_WIN; idx += BP_TX) {
+            const int j = idx / BP_WIN;
+            const int i = idx - j * BP_WIN;
+            float2 pr = make_float2(0.0f, 0.0f);
+            const int g = s_w0[j] + i;
+            if (s_cls[j] != 0 && g >= -1 && g <= proj - 1) {
+                const S* row = base + (size_t)s_row[j] * stride;
+                const int ga = max(g, 0);
+                const int gb = min(ga + 1, proj - 1);
+                float p = ld<S>(row + ga), q = ld<S>(row + gb);
+                if (p <= thr) p = kSentinel;
+                if (q <= thr) q = kSentinel;
+                if (g == -1) pr = make_float2(p - (q - p), q - p);
+                else if (g == proj - 1) pr = make_float2(p, 0.0f);
+                else pr = make_float2(p, q - p);
             }
-            const float c = cs[tr.tilt].x, s = cs[tr.tilt].y;
-            float umin = 0.5f * proj + fminf(c * fxt0, c * fxt1) + fminf(s * (float)fz0, s * fzt1);
-            int w0 = (int)floorf(umin) - 3;
-            const S* row = base + (size_t)tr.row * stride;
-            for (int i = tid; i < BP_WIN; i += BP_TX) {
-                int g = w0 + i;
-                float2 pr = make_float2(0.0f, 0.0f);
-                if (g >= -1 && g <= proj - 1) {
-                    int ga = max(g, 0);
-                    int gb = min(ga + 1, proj - 1);
-                    float p = ld<S>(row + ga), q = ld<S>(row + gb);
-                    if (p <= thr) p = kSentinel;
-                    if (q <= thr) q = kSentinel;
-                    if (g == -1) pr = make_float2(p - (q - p), q - p);
-                    else if (g == proj - 1) pr = make_float2(p, 0.0f);
-                    else pr = make_float2(p, q - p);
-                }
-                pairs[j][i] = pr;
-            }
-            if (tid == 0) win0[j] = w0;
+            pairs[j][i] = pr;
         }
         __syncthreads();
         // ---- accumulate ----
         for (int j = 0; j < nc; j++) {
-            const int w0 = win0[j];
-            if (w0 == INT_MIN) continue;
-            const TiltRow tr = list[c0 + j];
-            const uint8_t cl = cls[((size_t)tr.tilt * n_ztiles + zt) * n_xtiles + xt];
-            const float c = cs[tr.tilt].x, s = cs[tr.tilt].y;
-            const double kk = fz0 * (double)s + proj / 2.0 - (double)w0;
+            const int cl = s_cls[j];
+            if (cl == 0) continue;
+            const float c = s_cs[j].x, s = s_cs[j].y;
+            const double kk = fz0 * (double)s + proj / 2.0 - (double)s_w0[j];
             const float u0 = (float)(fx * (double)c + kk);
             const float2* pj = pairs[j];
             if (cl == 1 && !sent) {
@@ -253,10 +256,11 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
                     acc[k] += fmaf(f, pr.y, pr.x);
                 }
             } else {
+                const int tilt = s_tilt[j];
 #pragma unroll
                 for (int k = 0; k < BP_TZ; k++) {
                     const int z = min(z0 + k, vol_z - 1);
-                    const int2 r = lohi[(size_t)tr.tilt * vol_z + z];
+                    const int2 r = lohi[(size_t)tilt * vol_z + z];
                     float u = fmaf((float)k, s, u0);
                     int i = (int)u;
                     float f = __builtin_amdgcn_fractf(u);
@@ -292,76 +296,160 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
 
 // ------------------------------------------------------------------------
 // Forward projection + difference (Recon/GForProj.cu:10-76 and
-// Recon/GDiffProj.cu:7-21 fused).  One ray per lane, 256 consecutive
-// detector pixels of one tilt and slice per workgroup.  The FP64 tables give
-// each ray's sample interval, each 64-ray tile's union/core of intervals, and
-// per sample k the row offset, z weight and fzk*sin, so the inner loop is
-// FP32 bilinear interpolation with no bounds tests in the core.  The
-// difference raw - fwd (0 where either is empty) is written for the BP.
+// Recon/GDiffProj.cu:7-21 fused).
+//
+// One workgroup per (slice, group of tilts), one wave per (tilt, part of the
+// tilt's rays), RPL rays per lane (rays lane, lane+64, ...).  The slice is
+// streamed through LDS in z-bands of bz cell rows; cell (z, x) holds the
+// bilinear coefficients of the unit square above it,
+//   F = (V[z][x], V[z][x+1]-V[z][x], V[z+1][x]-V[z][x], d(x+1)-d(x) of both rows),
+// so a sample is a + f b + wz (c + f d): one ds_read_b128 and three FMAs.
+// Cell 0 of a row is a guard for x in [-1, 0) (linear extrapolation), which
+// lets the cell index be a plain truncation of x + 1.  The per-sample
+// geometry (cell row, z weight, fzk*sin) is one scalar load shared by the
+// RPL rays of every lane.  FP64 tables give, per (tilt, band), the samples
+// whose cell row lies in the band, and per (tilt, part, band) whether every
+// ray of the wave uses all of them; only the other bands test each sample
+// against the ray's own FP64 interval.  The difference raw - fwd (0 where
+// either is empty) is written for the back-projection.
 // ------------------------------------------------------------------------
-template <typename S>
-__global__ __launch_bounds__(FP_RAYS) void k_for_proj_diff(
+template <typename S, bool SENT, int RPL>
+__global__ __launch_bounds__(1024) void k_for_proj_diff(
     const float* __restrict__ vol, int vol_x, int vol_z, const int* __restrict__ vflag,
-    const int* __restrict__ fp_list, const float2* __restrict__ cs,
-    const int2* __restrict__ fp_range, const int4* __restrict__ fp_tile,
-    const int* __restrict__ geo_off, const float4* __restrict__ geo, int proj,
-    const S* __restrict__ sino, int n, int stride, S* __restrict__ diff, int diff_rows)
+    const int* __restrict__ fp_list, int n_list, int group, int wpt,
+    const float2* __restrict__ cs, const int2* __restrict__ fp_range,
+    const int2* __restrict__ fp_span, const uint8_t* __restrict__ fp_full,
+    const int* __restrict__ geo_off, const float4* __restrict__ geo,
+    const int2* __restrict__ fp_band, int nb, int bz, int proj, const S* __restrict__ sino, int n,
+    int stride, S* __restrict__ diff, int diff_rows)
 {
-    const int j = blockIdx.y, y = blockIdx.z;
+    extern __shared__ float4 band[];
+    const int y = blockIdx.y;
+    if ((vflag[y] != 0) != SENT) return;
+    const int tid = threadIdx.x, lane = tid & 63, nthr = blockDim.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int j0 = blockIdx.x * group;
+    const int ng = min(group, n_list - j0);
+    const int jw = wave / wpt, part = wave - jw * wpt;
+    const int j = j0 + min(jw, ng - 1);
     const int t = fp_list[j];
-    const int xd = blockIdx.x * FP_RAYS + threadIdx.x;
-    const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
-    const int wtile = min((blockIdx.x * FP_RAYS + (int)(threadIdx.x & ~(FP_WAVE - 1))) / FP_WAVE,
-                          ntile - 1);
-    const bool lane_ok = xd < proj;
-    const float c = cs[t].x;
-    const float tf = (float)(((double)xd - 0.5 * proj) / (double)c + vol_x * 0.5);
-    const int2 rr = lane_ok ? fp_range[(size_t)t * proj + xd] : make_int2(0, -1);
-    const int4 tl = fp_tile[(size_t)t * ntile + wtile];
+    const int2 sp = fp_span[t];
+    const int xa = sp.x + part * FP_WAVE * RPL;
+    const bool active = jw < ng && xa <= sp.y;
+    const int pitch = vol_x + 1;
+    const float xmax1 = (float)(vol_x - 1);
     const float* V = vol + (size_t)y * vol_z * vol_x;
     const float4* G = geo + geo_off[t];
-    const bool sent = vflag[y] != 0;
-    const float xmax = (float)(vol_x - 2);
+    const double c = (double)cs[t].x;
 
-    float sum = 0.0f;
-    int cnt = 0;
-    auto sample = [&](int k) -> float {
-        const float4 g = G[k];
-        float xs = __builtin_amdgcn_fmed3f(tf - g.z, 0.0f, xmax);
-        int ix = (int)xs;
-        float fxs = __builtin_amdgcn_fractf(xs);
-        const float* r0 = V + __float_as_int(g.x) + ix;
-        const float* r1 = r0 + vol_x;
-        float a = r0[0], b = r0[1], cc = r1[0], d = r1[1];
-        float top = fmaf(fxs, b - a, a);
-        float bot = fmaf(fxs, d - cc, cc);
-        return fmaf(g.y, bot - top, top);
-    };
-    if (!sent) {
-        for (int k = tl.x; k < tl.y; k++) {
-            float v = sample(k);
-            sum += (k >= rr.x && k <= rr.y) ? v : 0.0f;
+    float acc[RPL], tf1[RPL];
+    int cnt[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+        const int xd = min(xa + lane + FP_WAVE * r, sp.y);
+        tf1[r] = (float)(((double)xd - 0.5 * proj) / c + vol_x * 0.5 + 1.0);
+        acc[r] = 0.0f;
+        cnt[r] = 0;
+    }
+    for (int b = 0; b < nb; b++) {
+        const int zb = b * bz;
+        const int rows = min(bz, vol_z - 1 - zb);  // cell rows zb .. zb+rows-1 (<= VolZ-2)
+        for (int idx = tid; idx < rows * pitch; idx += nthr) {
+            const int r = idx / pitch;
+            const int xi = idx - r * pitch - 1;   // -1 .. VolX-1
+            const float* r0 = V + (size_t)(zb + r) * vol_x;
+            const float* r1 = r0 + vol_x;
+            float4 e;
+            if (xi < 0) {
+                float a = r0[0], dx = r0[1] - a, dz = r1[0] - a, dxz = (r1[1] - r1[0]) - dx;
+                e = make_float4(a - dx, dx, dz - dxz, dxz);
+            } else if (xi == vol_x - 1) {
+                float a = r0[xi];
+                e = make_float4(a, 0.0f, r1[xi] - a, 0.0f);
+            } else {
+                float a = r0[xi], c1 = r1[xi];
+                float dx = r0[xi + 1] - a;
+                e = make_float4(a, dx, c1 - a, (r1[xi + 1] - c1) - dx);
+            }
+            band[r * pitch + xi + 1] = e;
         }
-        for (int k = tl.y; k <= tl.z; k++) sum += sample(k);
-        for (int k = tl.z + 1; k <= tl.w; k++) {
-            float v = sample(k);
-            sum += (k >= rr.x && k <= rr.y) ? v : 0.0f;
+        __syncthreads();
+        if (active) {
+            const int2 kb = fp_band[(size_t)t * nb + b];
+            const bool full = !SENT && fp_full[((size_t)t * wpt + part) * nb + b] != 0;
+            if (kb.x <= kb.y && full) {
+                for (int k = kb.x; k <= kb.y; k++) {
+                    const float4 g = G[k];
+                    const float4* row = band + (__float_as_int(g.x) - zb) * pitch;
+#pragma unroll
+                    for (int r = 0; r < RPL; r++) {
+                        const float xs = tf1[r] - g.z;  // x + 1
+                        const int ix = (int)xs;
+                        const float f = __builtin_amdgcn_fractf(xs);
+                        const float4 e = row[ix];
+                        acc[r] += fmaf(g.y, fmaf(f, e.w, e.z), fmaf(f, e.y, e.x));
+                        // at most four cells in flight per lane: bounds VGPRs
+                        if ((r & 3) == 3) asm volatile("" ::: "memory");
+                    }
+                }
+            } else if (kb.x <= kb.y) {
+                // rays in blocks of four so that their intervals stay in VGPRs
+#pragma unroll
+                for (int r0 = 0; r0 < RPL; r0 += 4) {
+                    int2 rr[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int xd = xa + lane + FP_WAVE * (r0 + q);
+                        rr[q] = xd <= sp.y ? fp_range[(size_t)t * proj + xd] : make_int2(0, -1);
+                    }
+                    for (int k = kb.x; k <= kb.y; k++) {
+                        const float4 g = G[k];
+                        const float4* row = band + (__float_as_int(g.x) - zb) * pitch;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const int r = r0 + q;
+                            const float xs = __builtin_amdgcn_fmed3f(tf1[r] - g.z, 1.0f, xmax1);
+                            const int ix = (int)xs;
+                            const float f = __builtin_amdgcn_fractf(xs);
+                            const float4 e = row[ix];
+                            const float v = fmaf(g.y, fmaf(f, e.w, e.z), fmaf(f, e.y, e.x));
+                            bool in = k >= rr[q].x && k <= rr[q].y;
+                            if (SENT) in = in && !(v < kEmpty);
+                            acc[r] += in ? v : 0.0f;
+                            if (SENT) cnt[r] += in ? 1 : 0;
+                        }
+                    }
+                }
+            }
         }
-        cnt = max(rr.y - rr.x + 1, 0);
-    } else {
-        for (int k = tl.x; k <= tl.w; k++) {
-            float v = sample(k);
-            bool in = k >= rr.x && k <= rr.y && !(v < kEmpty);
-            sum += in ? v : 0.0f;
-            cnt += in ? 1 : 0;
+        __syncthreads();
+    }
+    if (jw >= ng) return;
+    // difference for the rays this wave owns
+    const S* raw_row = sino + ((size_t)y * n + t) * stride;
+    S* dst = diff + ((size_t)y * diff_rows + j) * stride;
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int xd = xa + lane + FP_WAVE * r;
+            if (xd <= sp.y) {
+                int count = cnt[r];
+                if (!SENT) {
+                    const int2 rr = fp_range[(size_t)t * proj + xd];
+                    count = max(rr.y - rr.x + 1, 0);
+                }
+                const float fwd = count <= 0 ? kSentinel : acc[r] / (float)count;
+                float raw = ld<S>(raw_row + xd);
+                if (raw <= empty_threshold<S>()) raw = kSentinel;
+                const float d = (fwd < kEmpty || raw < kEmpty) ? 0.0f : raw - fwd;
+                st<S>(dst + xd, d);
+            }
         }
     }
-    if (!lane_ok) return;
-    const float fwd = cnt <= 0 ? kSentinel : sum / (float)cnt;
-    float raw = ld<S>(sino + ((size_t)y * n + t) * stride + xd);
-    if (raw <= empty_threshold<S>()) raw = kSentinel;
-    float d = (fwd < kEmpty || raw < kEmpty) ? 0.0f : raw - fwd;
-    st<S>(diff + ((size_t)y * diff_rows + j) * stride + xd, d);
+    // rays that miss the slice: no sample, forward value empty, difference 0
+    if (part == 0)
+        for (int xd = lane; xd < proj; xd += FP_WAVE)
+            if (xd < sp.x || xd > sp.y) st<S>(dst + xd, 0.0f);
 }
 
 // ------------------------------------------------------------------------
@@ -461,6 +549,24 @@ __global__ __launch_bounds__(256) void k_output(const float* __restrict__ vol, i
     out[out_index(layout, x, y, z, vol_x, vol_y, vol_z)] = v;
 }
 
+// Plain streaming copy (16 B per lane, grid-stride): the calibration kernel
+// for HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src,
+                                              float4* __restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s)
+{
+    size_t n = bytes / sizeof(float4);
+    if (n == 0) return;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const float4*)src, (float4*)dst, n);
+}
+
 // ------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------
@@ -517,14 +623,40 @@ void launch_bp(const Plan& p, const void* sino, int rows, const TiltRow* list, i
     else launch_bp_t<float>(p, sino, rows, list, n_list, relax, check_sent, s);
 }
 
+template <typename S, bool SENT, int RPL>
+static void launch_fp_one(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
+{
+    const int wpt = p.geo.fp_wpt;
+    const int group = std::max(1, std::min(n_list, FP_MAX_WAVES / wpt));
+    dim3 grid((n_list + group - 1) / group, p.vol_y);
+    dim3 block(FP_WAVE * group * wpt);
+    size_t shm = (size_t)p.geo.fp_bz * (p.vol_x + 1) * sizeof(float4);
+    static bool attr_set = false;  // per instantiation: allow > 64 KiB of dynamic LDS
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)k_for_proj_diff<S, SENT, RPL>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, FP_LDS_BUDGET);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_for_proj_diff<S, SENT, RPL>), grid, block, shm, s, p.d_vol, p.vol_x,
+                       p.vol_z, p.d_vflag, fp_list, n_list, group, wpt, p.d_cs, p.d_fp_range,
+                       p.d_fp_span, p.d_fp_full, p.d_fp_geo_off, p.d_fp_geo, p.d_fp_band,
+                       p.geo.fp_nbands, p.geo.fp_bz, p.proj, (const S*)p.d_sino, p.n,
+                       p.sino_stride, (S*)p.d_diff, p.max_subset);
+}
+
+template <typename S, int RPL>
+static void launch_fp_rpl(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
+{
+    launch_fp_one<S, false, RPL>(p, fp_list, n_list, s);
+    launch_fp_one<S, true, RPL>(p, fp_list, n_list, s);
+}
+
 template <typename S>
 static void launch_fp_t(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
 {
-    dim3 grid((p.proj + FP_RAYS - 1) / FP_RAYS, n_list, p.vol_y);
-    hipLaunchKernelGGL(k_for_proj_diff<S>, grid, dim3(FP_RAYS), 0, s, p.d_vol, p.vol_x, p.vol_z,
-                       p.d_vflag, fp_list, p.d_cs, p.d_fp_range, p.d_fp_tile, p.d_fp_geo_off,
-                       p.d_fp_geo, p.proj, (const S*)p.d_sino, p.n, p.sino_stride, (S*)p.d_diff,
-                       p.max_subset);
+    if (p.geo.fp_rpl == 4) launch_fp_rpl<S, 4>(p, fp_list, n_list, s);
+    else if (p.geo.fp_rpl == 8) launch_fp_rpl<S, 8>(p, fp_list, n_list, s);
+    else launch_fp_rpl<S, 16>(p, fp_list, n_list, s);
 }
 
 void launch_fp_diff(const Plan& p, const int* fp_list, int n_list, hipStream_t s)

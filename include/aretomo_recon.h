@@ -174,6 +174,9 @@ atr_status atr_free(void* ptr);
 atr_status atr_memcpy_h2d(void* dst, const void* src, size_t bytes);
 atr_status atr_memcpy_d2h(void* dst, const void* src, size_t bytes);
 atr_status atr_memset(void* ptr, int value, size_t bytes);
+/* Plain streaming copy kernel (16-byte aligned), the reference workload for
+ * calibrating HBM byte counters. */
+atr_status atr_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
 atr_status atr_synchronize(void);
 atr_status atr_stream_create(void** stream);
 atr_status atr_stream_destroy(void* stream);
