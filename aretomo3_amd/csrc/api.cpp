@@ -124,7 +124,7 @@ static atr_status plan_sizes(const atr_series_desc* d, Plan& p)
         if (p.n_subsets < 1 || p.n_subsets > p.sart_n)
             return fail(ATR_ERR_INVALID_ARG, "sart_subsets must be in [1, number of SART tilts]");
         p.relax = d->sart_relax > 0 ? d->sart_relax : 1.0f / (float)p.n_subsets;
-        if ((p.vol_x + 1) * (int)sizeof(float4) > FP_LDS_BUDGET)
+        if (2 * (p.vol_x + 1) * (int)sizeof(float2) > FP_LDS_BUDGET)
             return fail(ATR_ERR_UNSUPPORTED_SIZE, "SART supports VolX <= 4991");
         // subset lists (Recon/CTomoSart.cpp:60-61,74-80; AreTomo/Util/CSplitItems.cpp)
         std::vector<int> starts, sizes;
@@ -246,12 +246,11 @@ static atr_status plan_build(Plan& p)
         if ((st = upload(&p.d_bp_lohi, g.bp_lohi, B)) != ATR_OK) return st;
         if ((st = upload(&p.d_bp_class, g.bp_class, B)) != ATR_OK) return st;
         if ((st = upload(&p.d_fp_range, g.fp_range, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_span, g.fp_span, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_full, g.fp_full, B)) != ATR_OK) return st;
         if ((st = upload(&p.d_fp_geo_off, g.fp_geo_off, B)) != ATR_OK) return st;
         if ((st = upload(&p.d_fp_len, g.fp_len, B)) != ATR_OK) return st;
         if ((st = upload(&p.d_fp_geo, g.fp_geo, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_band, g.fp_band, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_fp_desc, g.fp_desc, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_fp_pairs, fp_pair_table(p.proj), B)) != ATR_OK) return st;
         std::vector<TiltRow> lists(p.all_tilts);
         lists.insert(lists.end(), p.subset_tilts.begin(), p.subset_tilts.end());
         if ((st = upload(&p.d_lists, lists, B)) != ATR_OK) return st;
@@ -292,9 +291,8 @@ static atr_status plan_build(Plan& p)
 
 static void plan_free(Plan& p)
 {
-    void* ptrs[] = {p.d_cs, p.d_bp_lohi, p.d_bp_class, p.d_wbp_lohi, p.d_fp_range, p.d_fp_span,
-                    p.d_fp_full,
-                    p.d_fp_geo_off, p.d_fp_len, p.d_fp_geo, p.d_fp_band, p.d_lists, p.d_fp_lists,
+    void* ptrs[] = {p.d_cs, p.d_bp_lohi, p.d_bp_class, p.d_wbp_lohi, p.d_fp_range,
+                    p.d_fp_geo_off, p.d_fp_len, p.d_fp_geo, p.d_fp_desc, p.d_fp_pairs, p.d_lists, p.d_fp_lists,
                     p.d_fill_aln, p.d_fill_vol, p.d_filter, p.d_tw_proj, p.d_tw_x2, p.d_tw_z2,
                     p.d_tw_z, p.d_tw_x, p.d_axis_cs, p.d_shifts, p.d_excl, p.d_aligned,
                     p.d_row_stats, p.d_tilt_stats, p.d_sino, p.d_diff, p.d_vol, p.d_sflag,

@@ -23,13 +23,17 @@ constexpr int BP_TX = 256;
 constexpr int BP_TZ = 16;
 constexpr int BP_CHUNK = 16;                 // tilts staged in LDS at once
 constexpr int BP_WIN = BP_TX + BP_TZ + 16;   // staged pairs per tilt
-// Forward projection: a workgroup owns one slice and a group of tilts, one
-// wave per (tilt, part of its rays), fp_rpl rays per lane; the slice is
-// streamed through LDS in z-bands of bilinear cells, sized so that two
-// workgroups share a CU.
+// Forward projection: a workgroup of FP_THREADS owns one slice and a group of
+// tilts; wave w takes the (tilt, 64-ray tile) pairs w, w+16, ... (FP_MAXP at
+// most), so every wave sees every tilt; the slice is streamed through LDS in
+// z-bands of interpolation pairs.
+constexpr int FP_THREADS = 1024;
+constexpr int FP_WAVES = FP_THREADS / 64;
 constexpr int FP_WAVE = 64;
-constexpr int FP_MAX_WAVES = 16;
+constexpr int FP_MAXP = 16;
 constexpr int FP_LDS_BUDGET = 150 * 1024;
+constexpr int FP_BZ_MAX = 17;                // cell rows per band at most
+constexpr int FP_STAGE_ROWS = 6;             // rows a thread loads before forming pairs
 
 enum Kernel {
     K_BP = 0,
@@ -56,14 +60,12 @@ struct Geometry {
     std::vector<int2> bp_lohi;         // [N][VolZ] x-interval of GBackProj inclusion
     std::vector<uint8_t> bp_class;     // [N][ztiles][xtiles] 0 empty, 1 full, 2 partial
     std::vector<int2> wbp_lohi;        // [N][2VolZ] x2-interval of GBackProjWbp inclusion
-    std::vector<int2> fp_range;        // [N][P] valid sample interval of each ray
-    std::vector<int2> fp_span;         // [N] first/last ray that meets the slice
-    std::vector<uint8_t> fp_full;      // [N][wpt][nbands] band window used by every ray
-    int fp_rpl = 16, fp_wpt = 1;       // rays per lane, waves per tilt
+    std::vector<int2> fp_range;        // [N][P] first valid sample and count of each ray
     std::vector<int> fp_len;           // [N] ray length L
     std::vector<int> fp_geo_off;       // [N] offset into fp_geo
     std::vector<float4> fp_geo;        // [sum L] {row index bits, wz, q, 0}
     std::vector<int2> fp_band;         // [N][nbands] sample range whose rows lie in a z-band
+    std::vector<int4> fp_desc;         // [N][P/64][nbands] first geometry entry, count, first k, full
     int fp_bz = 0, fp_nbands = 0;      // z rows per band
 };
 
@@ -97,12 +99,11 @@ struct Plan {
     uint8_t* d_bp_class = nullptr;
     int2* d_wbp_lohi = nullptr;
     int2* d_fp_range = nullptr;
-    int2* d_fp_span = nullptr;
-    uint8_t* d_fp_full = nullptr;
     int* d_fp_geo_off = nullptr;
     int* d_fp_len = nullptr;
     float4* d_fp_geo = nullptr;
-    int2* d_fp_band = nullptr;
+    int4* d_fp_desc = nullptr;
+    int* d_fp_pairs = nullptr;
     TiltRow* d_lists = nullptr;       // all_tilts then every subset's BP list
     int* d_fp_lists = nullptr;        // every subset's FP list
     int* d_fill_aln = nullptr;        // [aln_x*aln_y]
@@ -162,6 +163,7 @@ void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                      int bz, Geometry& g);
 int fp_band_rows(int vol_x);
+std::vector<int> fp_pair_table(int proj);
 void split_items(int n_items, int n_splits, std::vector<int>& starts, std::vector<int>& sizes);
 void fill_table(uint64_t seed, int n, std::vector<int>& table);
 void aligned_size(int raw_x, int raw_y, float axis_deg, int* ax, int* ay);

@@ -172,10 +172,10 @@ void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol
 // Recon/GForProj.cu:131-149: ray (t, xd) has L = (int)(VolZ/cos + 1.5)
 // samples; sample k at fzk = k - L*0.5, x = ((xd - 0.5 ProjX)/cos + VolX*0.5)
 // - fzk sin, z = VolZ*0.5 + fzk cos, used iff inside [0,VolX-2]x[0,VolZ-2].
-// Tables: per ray its sample interval; per tilt the run of rays that meet
-// the slice; per (tilt, k) the cell row, the z weight and fzk*sin; per
-// (tilt, band) the samples whose cell row lies in the band; per (tilt, wave
-// part, band) whether every ray of the part uses the whole band window.
+// Tables: per ray its sample interval (as first sample and count); per
+// (tilt, k) the cell row, the z weight and fzk*sin; per (tilt, band) the
+// samples whose cell row lies in the band; per (tilt, 64-ray tile, band)
+// whether every ray of the tile uses the whole band window.
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                      int bz, Geometry& g)
 {
@@ -194,9 +194,9 @@ bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, 
         g.fp_geo_off[t] = (int)total;
         total += (size_t)len;
     }
-    g.fp_geo.assign(total, make_float4(0, 0, 0, 0));
-    g.fp_range.assign((size_t)n * proj, make_int2(0, -1));
-    g.fp_span.assign(n, make_int2(0, -1));
+    // four spare entries: the kernels read geometry in groups of four
+    g.fp_geo.assign(total + 4, make_float4(0, 0, 0, 0));
+    g.fp_range.assign((size_t)n * proj, make_int2(0, 0));
     const int nb = (vol_z - 1 + bz - 1) / bz;
     g.fp_bz = bz;
     g.fp_nbands = nb;
@@ -219,7 +219,10 @@ bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, 
             int iz = (int)zc;
             float wz = (float)(zc - iz);
             float q = (float)(fzk * s);
-            geo[k] = make_float4(__builtin_bit_cast(float, iz), wz, q, 0.0f);
+            // byte offset of the sample's lower row inside its band buffer
+            // (row iz % bz, (VolX + 1) interpolation pairs per row)
+            const int row_bytes = (iz % bz) * (vol_x + 1) * 8;
+            geo[k] = make_float4(__builtin_bit_cast(float, row_bytes), wz, q, 0.0f);
             if (zin) {
                 int2& bb = g.fp_band[(size_t)t * nb + iz / bz];
                 if (bb.y < bb.x) bb = make_int2(k, k);
@@ -242,46 +245,41 @@ bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, 
                     break;
                 }
             }
-            g.fp_range[(size_t)t * proj + xd] = lo < 0 ? make_int2(0, -1) : make_int2(lo, hi);
+            g.fp_range[(size_t)t * proj + xd] = lo < 0 ? make_int2(0, 0) : make_int2(lo, hi - lo + 1);
         }
-        // rays that meet the slice form one contiguous run (convex region)
-        int x0 = proj, x1 = -1;
-        for (int xd = 0; xd < proj; xd++) {
-            int2 r = g.fp_range[(size_t)t * proj + xd];
-            if (r.x <= r.y) {
-                x0 = std::min(x0, xd);
-                x1 = std::max(x1, xd);
-            }
-        }
-        g.fp_span[t] = x0 <= x1 ? make_int2(x0, x1) : make_int2(0, -1);
     });
-    // rays per lane / waves per tilt from the widest run
-    int span_max = 1;
-    for (int t = 0; t < n; t++) span_max = std::max(span_max, g.fp_span[t].y - g.fp_span[t].x + 1);
-    g.fp_rpl = span_max <= 4 * FP_WAVE ? 4 : (span_max <= 8 * FP_WAVE ? 8 : 16);
-    g.fp_wpt = (span_max + FP_WAVE * g.fp_rpl - 1) / (FP_WAVE * g.fp_rpl);
-    const int rays_per_wave = FP_WAVE * g.fp_rpl;
-    // full[t][part][band]: every ray the wave owns uses every sample of the
-    // band's window, so the wave runs the band without per-sample tests
-    g.fp_full.assign((size_t)n * g.fp_wpt * nb, 0);
+    // One descriptor per (tilt, 64-ray tile, band): the band's sample window
+    // clipped to the union of the tile's ray intervals, as {first geometry
+    // entry, sample count, first sample index, 1 if every ray of the tile
+    // uses every sample of the window}.
+    g.fp_desc.assign((size_t)n * ntile * nb, make_int4(0, 0, 0, 0));
     parallel_tilts(n, [&](int t) {
-        const int2 sp = g.fp_span[t];
-        for (int part = 0; part < g.fp_wpt; part++) {
-            const int a = sp.x + part * rays_per_wave;
-            const int e = std::min(sp.y, a + rays_per_wave - 1);
+        for (int tile = 0; tile < ntile; tile++) {
+            const int a = tile * FP_WAVE, e = std::min(proj, a + FP_WAVE) - 1;
+            int ulo = 1 << 30, uhi = -1;
+            for (int xd = a; xd <= e; xd++) {
+                int2 r = g.fp_range[(size_t)t * proj + xd];
+                if (r.y > 0) {
+                    ulo = std::min(ulo, r.x);
+                    uhi = std::max(uhi, r.x + r.y - 1);
+                }
+            }
             for (int b = 0; b < nb; b++) {
                 const int2 kb = g.fp_band[(size_t)t * nb + b];
-                bool full = true;
-                if (kb.x <= kb.y)
+                const int lo = std::max(kb.x, ulo), hi = std::min(kb.y, uhi);
+                int4 d = make_int4(0, 0, 0, 0);
+                if (kb.x <= kb.y && lo <= hi) {
+                    bool full = true;
                     for (int xd = a; xd <= e && full; xd++) {
                         int2 r = g.fp_range[(size_t)t * proj + xd];
-                        full = r.x <= kb.x && r.y >= kb.y;
+                        full = r.y > 0 && r.x <= lo && r.x + r.y - 1 >= hi;
                     }
-                g.fp_full[((size_t)t * g.fp_wpt + part) * nb + b] = full ? 1 : 0;
+                    d = make_int4(g.fp_geo_off[t] + lo, hi - lo + 1, lo, full ? 1 : 0);
+                }
+                g.fp_desc[((size_t)t * ntile + tile) * nb + b] = d;
             }
         }
     });
-    (void)ntile;
     return true;
 }
 
@@ -327,11 +325,21 @@ std::vector<float> ramp_filter(int n)
 }  // namespace atr
 
 namespace atr {
-// Cell rows per forward-projection band: bz rows of (VolX + 1) float4 cells
-// must fit the LDS budget of one workgroup.
+// Cell rows per forward-projection band: bz + 1 rows of (VolX + 1)
+// interpolation pairs must fit the LDS budget of one workgroup.
 int fp_band_rows(int vol_x)
 {
-    int rows = FP_LDS_BUDGET / ((vol_x + 1) * (int)sizeof(float4));
-    return std::min(32, std::max(1, rows));
+    int rows = FP_LDS_BUDGET / ((vol_x + 1) * (int)sizeof(float2));
+    return std::min(FP_BZ_MAX, std::max(1, rows - 1));
+}
+
+// (tilt-in-group, tile) of every pair index p = wave + 16 i of the forward
+// projector: j << 16 | tile.
+std::vector<int> fp_pair_table(int proj)
+{
+    const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
+    std::vector<int> tab(FP_WAVES * FP_MAXP);
+    for (int p = 0; p < (int)tab.size(); p++) tab[p] = ((p / ntile) << 16) | (p % ntile);
+    return tab;
 }
 }  // namespace atr

@@ -298,158 +298,212 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
 // Forward projection + difference (Recon/GForProj.cu:10-76 and
 // Recon/GDiffProj.cu:7-21 fused).
 //
-// One workgroup per (slice, group of tilts), one wave per (tilt, part of the
-// tilt's rays), RPL rays per lane (rays lane, lane+64, ...).  The slice is
+// One workgroup (FP_THREADS) per (slice, group of tilts).  The slice is
 // streamed through LDS in z-bands of bz cell rows; cell (z, x) holds the
 // bilinear coefficients of the unit square above it,
 //   F = (V[z][x], V[z][x+1]-V[z][x], V[z+1][x]-V[z][x], d(x+1)-d(x) of both rows),
 // so a sample is a + f b + wz (c + f d): one ds_read_b128 and three FMAs.
 // Cell 0 of a row is a guard for x in [-1, 0) (linear extrapolation), which
-// lets the cell index be a plain truncation of x + 1.  The per-sample
-// geometry (cell row, z weight, fzk*sin) is one scalar load shared by the
-// RPL rays of every lane.  FP64 tables give, per (tilt, band), the samples
-// whose cell row lies in the band, and per (tilt, part, band) whether every
-// ray of the wave uses all of them; only the other bands test each sample
-// against the ray's own FP64 interval.  The difference raw - fwd (0 where
-// either is empty) is written for the back-projection.
+// lets the cell index be a plain truncation of x + 1.  Wave w owns the
+// (tilt, 64-ray tile) pairs w, w+16, ..., one ray per lane and one FP32
+// accumulator per pair, so every wave carries the same mix of tilts and the
+// per-band barrier does not leave waves idle.  FP64 tables give, per (tilt,
+// band), the samples whose cell row lies in the band and, per (tilt, tile,
+// band), whether every ray of the tile uses all of them; only the other
+// blocks test each sample against the ray's FP64 interval.  Samples are
+// processed four at a time with their geometry and cells loaded before any
+// is used.  The difference raw - fwd (0 where either is empty) is written for
+// the back-projection.
 // ------------------------------------------------------------------------
-template <typename S, bool SENT, int RPL>
-__global__ __launch_bounds__(1024) void k_for_proj_diff(
-    const float* __restrict__ vol, int vol_x, int vol_z, const int* __restrict__ vflag,
-    const int* __restrict__ fp_list, int n_list, int group, int wpt,
-    const float2* __restrict__ cs, const int2* __restrict__ fp_range,
-    const int2* __restrict__ fp_span, const uint8_t* __restrict__ fp_full,
-    const int* __restrict__ geo_off, const float4* __restrict__ geo,
-    const int2* __restrict__ fp_band, int nb, int bz, int proj, const S* __restrict__ sino, int n,
-    int stride, S* __restrict__ diff, int diff_rows)
+// Geometry of four consecutive samples, fetched with one 64-byte scalar load.
+struct Geo4 {
+    float4 g[4];
+};
+
+// Bilinear sample from the band: g.x is the byte offset of the sample's lower
+// row in the band buffer, xs = x + 1 (the +1 skips the guard pair), rowb the
+// byte pitch of a band row.
+__device__ __forceinline__ float fp_sample(const char* __restrict__ band, float4 g, float xs,
+                                           int rowb)
 {
-    extern __shared__ float4 band[];
+    const int ix = (int)xs;
+    const float f = __builtin_amdgcn_fractf(xs);
+    const char* p = band + __float_as_int(g.x) + 8 * ix;
+    const float2 e0 = *(const float2*)p;
+    const float2 e1 = *(const float2*)(p + rowb);
+    const float top = fmaf(f, e0.y, e0.x);
+    const float bot = fmaf(f, e1.y, e1.x);
+    return fmaf(g.y, bot - top, top);
+}
+
+// Samples [0, count) of one (pair, band): geometry entries G[0..count),
+// sample indices k0 + m.  Groups of four run without tests; the final
+// partial group clamps and drops the samples past the window.  MASKED also
+// tests each sample against the ray's FP64 interval rr = (first, count).
+template <bool MASKED, bool SENT>
+__device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
+                                       const float4* __restrict__ G, int count, int k0,
+                                       float tf1, float xmax1, int2 rr, float& acc, int& cnt)
+{
+    int m = 0;
+    for (; m + 4 <= count; m += 4) {
+        const Geo4 g4 = *(const Geo4*)(G + m);
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            float xs = tf1 - g4.g[q].z;  // x + 1
+            if (MASKED) xs = __builtin_amdgcn_fmed3f(xs, 1.0f, xmax1);
+            v[q] = fp_sample(band, g4.g[q], xs, rowb);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (MASKED) {
+                bool in = (unsigned)(k0 + m + q - rr.x) < (unsigned)rr.y;
+                if (SENT) in = in & !(v[q] < kEmpty);
+                acc += in ? v[q] : 0.0f;
+                if (SENT) cnt += in ? 1 : 0;
+            } else {
+                acc += v[q];
+            }
+        }
+    }
+    if (m < count) {
+        const Geo4 g4 = *(const Geo4*)(G + m);
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            // entries past the window may belong to another band: read row 0
+            const float4 g = m + q < count ? g4.g[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float xs = __builtin_amdgcn_fmed3f(tf1 - g4.g[q].z, 1.0f, xmax1);
+            v[q] = fp_sample(band, g, xs, rowb);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            bool in = m + q < count;
+            if (MASKED) in = in & ((unsigned)(k0 + m + q - rr.x) < (unsigned)rr.y);
+            if (SENT) in = in & !(v[q] < kEmpty);
+            acc += in ? v[q] : 0.0f;
+            if (SENT) cnt += in ? 1 : 0;
+        }
+    }
+}
+
+template <typename S, bool SENT>
+__global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(
+    const float* __restrict__ vol, int vol_x, int vol_z, const int* __restrict__ vflag,
+    const int* __restrict__ fp_list, int n_list, int group, const int* __restrict__ pair_tab,
+    const float2* __restrict__ cs, const int2* __restrict__ fp_range,
+    const float4* __restrict__ geo, const int4* __restrict__ fp_desc, int nb, int bz, int proj,
+    const S* __restrict__ sino, int n, int stride, S* __restrict__ diff, int diff_rows)
+{
+    extern __shared__ float2 band[];
     const int y = blockIdx.y;
     if ((vflag[y] != 0) != SENT) return;
-    const int tid = threadIdx.x, lane = tid & 63, nthr = blockDim.x;
+    const int tid = threadIdx.x, lane = tid & 63;
+    // wave index made provably uniform so that all per-pair table reads are
+    // scalar loads into SGPRs
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int j0 = blockIdx.x * group;
     const int ng = min(group, n_list - j0);
-    const int jw = wave / wpt, part = wave - jw * wpt;
-    const int j = j0 + min(jw, ng - 1);
-    const int t = fp_list[j];
-    const int2 sp = fp_span[t];
-    const int xa = sp.x + part * FP_WAVE * RPL;
-    const bool active = jw < ng && xa <= sp.y;
-    const int pitch = vol_x + 1;
-    const float xmax1 = (float)(vol_x - 1);
+    const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
+    const int npairs = ng * ntile;
+    const int pitch = vol_x + 1;               // pairs per band row (guard + VolX)
+    const int rowb = pitch * (int)sizeof(float2);
+    const float xmax1 = (float)(vol_x - 1);    // clamp of x + 1 for masked samples
     const float* V = vol + (size_t)y * vol_z * vol_x;
-    const float4* G = geo + geo_off[t];
-    const double c = (double)cs[t].x;
-
-    float acc[RPL], tf1[RPL];
-    int cnt[RPL];
+    float acc[FP_MAXP], tf1[FP_MAXP];
+    int cnt[FP_MAXP];
 #pragma unroll
-    for (int r = 0; r < RPL; r++) {
-        const int xd = min(xa + lane + FP_WAVE * r, sp.y);
-        tf1[r] = (float)(((double)xd - 0.5 * proj) / c + vol_x * 0.5 + 1.0);
-        acc[r] = 0.0f;
-        cnt[r] = 0;
+    for (int i = 0; i < FP_MAXP; i++) {
+        acc[i] = 0.0f;
+        cnt[i] = 0;
+        tf1[i] = 1.0f;
+        const int p = wave + FP_WAVES * i;
+        if (p < npairs) {
+            const int jt = pair_tab[p];
+            const int t = fp_list[j0 + (jt >> 16)];
+            const int xd = min((jt & 0xffff) * FP_WAVE + lane, proj - 1);
+            tf1[i] = (float)(((double)xd - 0.5 * proj) / (double)cs[t].x + vol_x * 0.5 + 1.0);
+        }
     }
+    const char* bandb = (const char*)band;
     for (int b = 0; b < nb; b++) {
         const int zb = b * bz;
-        const int rows = min(bz, vol_z - 1 - zb);  // cell rows zb .. zb+rows-1 (<= VolZ-2)
-        for (int idx = tid; idx < rows * pitch; idx += nthr) {
-            const int r = idx / pitch;
-            const int xi = idx - r * pitch - 1;   // -1 .. VolX-1
-            const float* r0 = V + (size_t)(zb + r) * vol_x;
-            const float* r1 = r0 + vol_x;
-            float4 e;
-            if (xi < 0) {
-                float a = r0[0], dx = r0[1] - a, dz = r1[0] - a, dxz = (r1[1] - r1[0]) - dx;
-                e = make_float4(a - dx, dx, dz - dxz, dxz);
-            } else if (xi == vol_x - 1) {
-                float a = r0[xi];
-                e = make_float4(a, 0.0f, r1[xi] - a, 0.0f);
-            } else {
-                float a = r0[xi], c1 = r1[xi];
-                float dx = r0[xi + 1] - a;
-                e = make_float4(a, dx, c1 - a, (r1[xi + 1] - c1) - dx);
-            }
-            band[r * pitch + xi + 1] = e;
-        }
-        __syncthreads();
-        if (active) {
-            const int2 kb = fp_band[(size_t)t * nb + b];
-            const bool full = !SENT && fp_full[((size_t)t * wpt + part) * nb + b] != 0;
-            if (kb.x <= kb.y && full) {
-                for (int k = kb.x; k <= kb.y; k++) {
-                    const float4 g = G[k];
-                    const float4* row = band + (__float_as_int(g.x) - zb) * pitch;
+        const int rows = min(bz, vol_z - 1 - zb) + 1;  // volume rows zb .. zb+rows-1
+        // Staging: a thread owns whole columns of interpolation pairs
+        // (V[z][x], V[z][x+1]-V[z][x]); column 0 is the guard for x in [-1, 0).
+        // Each half of the rows is loaded first (independent, coalesced
+        // loads), then the pairs are formed from registers.
+        for (int col = tid; col < pitch; col += FP_THREADS) {
+            const int xi = col - 1;                      // -1 .. VolX-1
+            const int xa = max(xi, 0), xb = min(xa + 1, vol_x - 1);
+            const float* base = V + (size_t)zb * vol_x;
 #pragma unroll
-                    for (int r = 0; r < RPL; r++) {
-                        const float xs = tf1[r] - g.z;  // x + 1
-                        const int ix = (int)xs;
-                        const float f = __builtin_amdgcn_fractf(xs);
-                        const float4 e = row[ix];
-                        acc[r] += fmaf(g.y, fmaf(f, e.w, e.z), fmaf(f, e.y, e.x));
-                        // at most four cells in flight per lane: bounds VGPRs
-                        if ((r & 3) == 3) asm volatile("" ::: "memory");
+            for (int h = 0; h < FP_BZ_MAX + 1; h += FP_STAGE_ROWS) {
+                float va[FP_STAGE_ROWS], vb[FP_STAGE_ROWS];
+#pragma unroll
+                for (int r = 0; r < FP_STAGE_ROWS; r++) {
+                    if (h + r < rows) {
+                        va[r] = base[(size_t)(h + r) * vol_x + xa];
+                        vb[r] = base[(size_t)(h + r) * vol_x + xb];
                     }
                 }
-            } else if (kb.x <= kb.y) {
-                // rays in blocks of four so that their intervals stay in VGPRs
 #pragma unroll
-                for (int r0 = 0; r0 < RPL; r0 += 4) {
-                    int2 rr[4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int xd = xa + lane + FP_WAVE * (r0 + q);
-                        rr[q] = xd <= sp.y ? fp_range[(size_t)t * proj + xd] : make_int2(0, -1);
+                for (int r = 0; r < FP_STAGE_ROWS; r++) {
+                    if (h + r < rows) {
+                        const float dx = vb[r] - va[r];
+                        band[(h + r) * pitch + col] =
+                            xi < 0 ? make_float2(va[r] - dx, dx) : make_float2(va[r], dx);
                     }
-                    for (int k = kb.x; k <= kb.y; k++) {
-                        const float4 g = G[k];
-                        const float4* row = band + (__float_as_int(g.x) - zb) * pitch;
+                }
+            }
+        }
+        __syncthreads();
 #pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                            const int r = r0 + q;
-                            const float xs = __builtin_amdgcn_fmed3f(tf1[r] - g.z, 1.0f, xmax1);
-                            const int ix = (int)xs;
-                            const float f = __builtin_amdgcn_fractf(xs);
-                            const float4 e = row[ix];
-                            const float v = fmaf(g.y, fmaf(f, e.w, e.z), fmaf(f, e.y, e.x));
-                            bool in = k >= rr[q].x && k <= rr[q].y;
-                            if (SENT) in = in && !(v < kEmpty);
-                            acc[r] += in ? v : 0.0f;
-                            if (SENT) cnt[r] += in ? 1 : 0;
-                        }
+        for (int i = 0; i < FP_MAXP; i++) {
+            const int p = wave + FP_WAVES * i;
+            if (p < npairs) {
+                const int jt = pair_tab[p];
+                const int t = fp_list[j0 + (jt >> 16)];
+                const int tile = jt & 0xffff;
+                const int4 d = fp_desc[((size_t)t * ntile + tile) * nb + b];
+                if (d.y > 0) {
+                    const float4* G = geo + d.x;
+                    if (!SENT && d.w != 0) {
+                        fp_run<false, false>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1,
+                                             make_int2(0, 0), acc[i], cnt[i]);
+                    } else {
+                        int xd = tile * FP_WAVE + lane;
+                        // keep the interval load inside the band loop: hoisted,
+                        // the intervals of all pairs would pin 2*FP_MAXP VGPRs
+                        asm volatile("" : "+v"(xd));
+                        const int2 rr = xd < proj ? fp_range[(size_t)t * proj + xd] : make_int2(0, 0);
+                        fp_run<true, SENT>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1, rr, acc[i],
+                                           cnt[i]);
                     }
                 }
             }
         }
         __syncthreads();
     }
-    if (jw >= ng) return;
-    // difference for the rays this wave owns
-    const S* raw_row = sino + ((size_t)y * n + t) * stride;
-    S* dst = diff + ((size_t)y * diff_rows + j) * stride;
-    if (active) {
 #pragma unroll
-        for (int r = 0; r < RPL; r++) {
-            const int xd = xa + lane + FP_WAVE * r;
-            if (xd <= sp.y) {
-                int count = cnt[r];
-                if (!SENT) {
-                    const int2 rr = fp_range[(size_t)t * proj + xd];
-                    count = max(rr.y - rr.x + 1, 0);
-                }
-                const float fwd = count <= 0 ? kSentinel : acc[r] / (float)count;
-                float raw = ld<S>(raw_row + xd);
+    for (int i = 0; i < FP_MAXP; i++) {
+        const int p = wave + FP_WAVES * i;
+        if (p < npairs) {
+            const int jt = pair_tab[p];
+            const int j = j0 + (jt >> 16);
+            const int t = fp_list[j];
+            const int xd = (jt & 0xffff) * FP_WAVE + lane;
+            if (xd < proj) {
+                const int count = SENT ? cnt[i] : fp_range[(size_t)t * proj + xd].y;
+                const float fwd = count <= 0 ? kSentinel : acc[i] / (float)count;
+                float raw = ld<S>(sino + ((size_t)y * n + t) * stride + xd);
                 if (raw <= empty_threshold<S>()) raw = kSentinel;
                 const float d = (fwd < kEmpty || raw < kEmpty) ? 0.0f : raw - fwd;
-                st<S>(dst + xd, d);
+                st<S>(diff + ((size_t)y * diff_rows + j) * stride + xd, d);
             }
         }
     }
-    // rays that miss the slice: no sample, forward value empty, difference 0
-    if (part == 0)
-        for (int xd = lane; xd < proj; xd += FP_WAVE)
-            if (xd < sp.x || xd > sp.y) st<S>(dst + xd, 0.0f);
 }
 
 // ------------------------------------------------------------------------
@@ -623,40 +677,30 @@ void launch_bp(const Plan& p, const void* sino, int rows, const TiltRow* list, i
     else launch_bp_t<float>(p, sino, rows, list, n_list, relax, check_sent, s);
 }
 
-template <typename S, bool SENT, int RPL>
+template <typename S, bool SENT>
 static void launch_fp_one(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
 {
-    const int wpt = p.geo.fp_wpt;
-    const int group = std::max(1, std::min(n_list, FP_MAX_WAVES / wpt));
+    const int ntile = (p.proj + FP_WAVE - 1) / FP_WAVE;
+    const int group = std::max(1, FP_WAVES * FP_MAXP / ntile);
     dim3 grid((n_list + group - 1) / group, p.vol_y);
-    dim3 block(FP_WAVE * group * wpt);
-    size_t shm = (size_t)p.geo.fp_bz * (p.vol_x + 1) * sizeof(float4);
+    size_t shm = (size_t)(p.geo.fp_bz + 1) * (p.vol_x + 1) * sizeof(float2);
     static bool attr_set = false;  // per instantiation: allow > 64 KiB of dynamic LDS
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_for_proj_diff<S, SENT, RPL>,
+        (void)hipFuncSetAttribute((const void*)k_for_proj_diff<S, SENT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, FP_LDS_BUDGET);
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_for_proj_diff<S, SENT, RPL>), grid, block, shm, s, p.d_vol, p.vol_x,
-                       p.vol_z, p.d_vflag, fp_list, n_list, group, wpt, p.d_cs, p.d_fp_range,
-                       p.d_fp_span, p.d_fp_full, p.d_fp_geo_off, p.d_fp_geo, p.d_fp_band,
-                       p.geo.fp_nbands, p.geo.fp_bz, p.proj, (const S*)p.d_sino, p.n,
-                       p.sino_stride, (S*)p.d_diff, p.max_subset);
-}
-
-template <typename S, int RPL>
-static void launch_fp_rpl(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
-{
-    launch_fp_one<S, false, RPL>(p, fp_list, n_list, s);
-    launch_fp_one<S, true, RPL>(p, fp_list, n_list, s);
+    hipLaunchKernelGGL((k_for_proj_diff<S, SENT>), grid, dim3(FP_THREADS), shm, s, p.d_vol,
+                       p.vol_x, p.vol_z, p.d_vflag, fp_list, n_list, group, p.d_fp_pairs, p.d_cs,
+                       p.d_fp_range, p.d_fp_geo, p.d_fp_desc, p.geo.fp_nbands, p.geo.fp_bz, p.proj,
+                       (const S*)p.d_sino, p.n, p.sino_stride, (S*)p.d_diff, p.max_subset);
 }
 
 template <typename S>
 static void launch_fp_t(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
 {
-    if (p.geo.fp_rpl == 4) launch_fp_rpl<S, 4>(p, fp_list, n_list, s);
-    else if (p.geo.fp_rpl == 8) launch_fp_rpl<S, 8>(p, fp_list, n_list, s);
-    else launch_fp_rpl<S, 16>(p, fp_list, n_list, s);
+    launch_fp_one<S, false>(p, fp_list, n_list, s);
+    launch_fp_one<S, true>(p, fp_list, n_list, s);
 }
 
 void launch_fp_diff(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
