@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libatr_recon.so")
 
 ATR_OK = 0
+ATR_ERR_INVALID_ARG, ATR_ERR_OUT_OF_MEMORY, ATR_ERR_HIP, ATR_ERR_UNSUPPORTED_SIZE = 1, 2, 3, 4
 STATUS_NAMES = {0: "ATR_OK", 1: "ATR_ERR_INVALID_ARG", 2: "ATR_ERR_OUT_OF_MEMORY",
                 3: "ATR_ERR_HIP", 4: "ATR_ERR_UNSUPPORTED_SIZE"}
 
