@@ -240,11 +240,11 @@ static atr_status plan_build(Plan& p)
 
     if (p.method == ATR_METHOD_SART) {
         build_bp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.bp_lohi);
-        build_bp_classes(g.bp_lohi, p.n, p.vol_x, p.vol_z, g.bp_class);
+        build_bp_tiles(g.cos_sin, g.bp_lohi, p.n, p.vol_x, p.vol_z, p.proj, g.bp_tiles);
         if (!build_fp_tables(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, fp_band_rows(p.vol_x), g))
             return fail(ATR_ERR_INVALID_ARG, "SART needs every tilt in (-90, 90) degrees");
         if ((st = upload(&p.d_bp_lohi, g.bp_lohi, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_bp_class, g.bp_class, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_bp_tiles, g.bp_tiles, B)) != ATR_OK) return st;
         if ((st = upload(&p.d_fp_range, g.fp_range, B)) != ATR_OK) return st;
         if ((st = upload(&p.d_fp_geo_off, g.fp_geo_off, B)) != ATR_OK) return st;
         if ((st = upload(&p.d_fp_len, g.fp_len, B)) != ATR_OK) return st;
@@ -291,7 +291,7 @@ static atr_status plan_build(Plan& p)
 
 static void plan_free(Plan& p)
 {
-    void* ptrs[] = {p.d_cs, p.d_bp_lohi, p.d_bp_class, p.d_wbp_lohi, p.d_fp_range,
+    void* ptrs[] = {p.d_cs, p.d_bp_lohi, p.d_bp_tiles, p.d_wbp_lohi, p.d_fp_range,
                     p.d_fp_geo_off, p.d_fp_len, p.d_fp_geo, p.d_fp_desc, p.d_fp_pairs, p.d_lists, p.d_fp_lists,
                     p.d_fill_aln, p.d_fill_vol, p.d_filter, p.d_tw_proj, p.d_tw_x2, p.d_tw_z2,
                     p.d_tw_z, p.d_tw_x, p.d_axis_cs, p.d_shifts, p.d_excl, p.d_aligned,

@@ -58,7 +58,7 @@ struct TiltRow {
 struct Geometry {
     std::vector<float> cos_sin;        // [2N] float, as CTomoBase.cpp:55-65
     std::vector<int2> bp_lohi;         // [N][VolZ] x-interval of GBackProj inclusion
-    std::vector<uint8_t> bp_class;     // [N][ztiles][xtiles] 0 empty, 1 full, 2 partial
+    std::vector<int2> bp_tiles;        // [N][ztiles][xtiles] {window start, class 0 empty / 1 full / 2 partial}
     std::vector<int2> wbp_lohi;        // [N][2VolZ] x2-interval of GBackProjWbp inclusion
     std::vector<int2> fp_range;        // [N][P] first valid sample and count of each ray
     std::vector<int> fp_len;           // [N] ray length L
@@ -96,7 +96,7 @@ struct Plan {
     // device tables
     float2* d_cs = nullptr;
     int2* d_bp_lohi = nullptr;
-    uint8_t* d_bp_class = nullptr;
+    int2* d_bp_tiles = nullptr;
     int2* d_wbp_lohi = nullptr;
     int2* d_fp_range = nullptr;
     int* d_fp_geo_off = nullptr;
@@ -156,8 +156,8 @@ atr_status hip_fail(hipError_t e, const char* what);
 void cos_sin_table(const float* tilts, int n, std::vector<float>& cs);
 void build_bp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                         std::vector<int2>& lohi);
-void build_bp_classes(const std::vector<int2>& lohi, int n, int vol_x, int vol_z,
-                      std::vector<uint8_t>& cls);
+void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi, int n, int vol_x,
+                    int vol_z, int proj, std::vector<int2>& tiles);
 void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                          std::vector<int2>& lohi);
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,

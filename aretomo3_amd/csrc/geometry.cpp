@@ -126,14 +126,19 @@ void build_bp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_
     });
 }
 
-// Tile class of every (tilt, z-tile, x-tile) of the SART back-projector:
-// 1 = every voxel of the tile uses the tilt, 0 = none does, 2 = some do.
-void build_bp_classes(const std::vector<int2>& lohi, int n, int vol_x, int vol_z,
-                      std::vector<uint8_t>& cls)
+// Per (tilt, z-tile, x-tile) of the SART back-projector: the start w0 of the
+// window of BP_WIN row samples the tile stages, and the tile class
+// (1 = every voxel of the tile uses the tilt, 0 = none does, 2 = some do).
+// w0 = floor(min u over the tile corners) - 3 with u = fx cos + fz sin + P/2
+// in FP64; the kernel's FP32 coordinates relative to w0 then stay in
+// [2, BP_WIN - 2] (the tile spans at most (BP_TX-1)|cos| + (BP_TZ-1)|sin| in u).
+void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi, int n, int vol_x,
+                    int vol_z, int proj, std::vector<int2>& tiles)
 {
     int nzt = (vol_z + BP_TZ - 1) / BP_TZ, nxt = (vol_x + BP_TX - 1) / BP_TX;
-    cls.assign((size_t)n * nzt * nxt, 0);
-    for (int t = 0; t < n; t++)
+    tiles.assign((size_t)n * nzt * nxt, int2{0, 0});
+    for (int t = 0; t < n; t++) {
+        const double c = cs[2 * t], s = cs[2 * t + 1];
         for (int zt = 0; zt < nzt; zt++)
             for (int xt = 0; xt < nxt; xt++) {
                 int x0 = xt * BP_TX, x1 = std::min(x0 + BP_TX, vol_x) - 1;
@@ -143,8 +148,13 @@ void build_bp_classes(const std::vector<int2>& lohi, int n, int vol_x, int vol_z
                     if (!(r.x <= x0 && r.y >= x1)) full = false;
                     if (!(r.x > x1 || r.y < x0)) any = true;
                 }
-                cls[((size_t)t * nzt + zt) * nxt + xt] = full ? 1 : (any ? 2 : 0);
+                const double fx0 = x0 - vol_x * 0.5, fx1 = x1 - vol_x * 0.5;
+                const double fz0 = zt * BP_TZ - vol_z * 0.5, fz1 = fz0 + (BP_TZ - 1);
+                const double umin = proj / 2.0 + std::min(c * fx0, c * fx1) + std::min(s * fz0, s * fz1);
+                tiles[((size_t)t * nzt + zt) * nxt + xt] =
+                    int2{(int)std::floor(umin) - 3, full ? 1 : (any ? 2 : 0)};
             }
+    }
 }
 
 // Recon/GBackProjWbp.cu:23-36 on the 2x grid (vol_x2 = 2 VolX, vol_z2 =

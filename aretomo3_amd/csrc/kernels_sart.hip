@@ -165,26 +165,30 @@ __global__ __launch_bounds__(256) void k_weight(const float* __restrict__ in, si
 // from the FP64 interval tables: tiles the host classified as fully inside
 // skip all per-lane tests and count per tile; partial tiles test x against
 // the (tilt, z) interval.  Sums and counts are FP32/int per voxel, in tilt
-// order, as the reference.
+// order, as the reference.  Everything per tilt (list entry, cos/sin, the
+// tile's window start and class) is wave-uniform and read with scalar loads.
 // ------------------------------------------------------------------------
 template <typename S>
 __global__ __launch_bounds__(BP_TX) void k_back_proj(
     const S* __restrict__ sino, int rows_per_slice, int stride, int proj,
     const TiltRow* __restrict__ list, int n_list, const float2* __restrict__ cs,
-    const int2* __restrict__ lohi, const uint8_t* __restrict__ cls, int n_ztiles, int n_xtiles,
+    const int2* __restrict__ lohi, const int2* __restrict__ tiles, int n_ztiles, int n_xtiles,
     float relax, int positivity, const int* __restrict__ sflag, float* __restrict__ vol,
     int vol_x, int vol_z, int* __restrict__ vflag)
 {
     __shared__ float2 pairs[BP_CHUNK][BP_WIN];
-    __shared__ int s_w0[BP_CHUNK], s_row[BP_CHUNK], s_tilt[BP_CHUNK], s_cls[BP_CHUNK];
-    __shared__ float2 s_cs[BP_CHUNK];
     const int tid = threadIdx.x;
     const int xt = blockIdx.x, zt = blockIdx.y, y = blockIdx.z;
     const int x0 = xt * BP_TX, z0 = zt * BP_TZ;
     const int x = x0 + tid;
     const int xc = min(x, vol_x - 1);
     const bool sent = sflag != nullptr && sflag[y] != 0;
-    const float thr = empty_threshold<S>();
+    float* vslice = vol + (size_t)y * vol_z * vol_x;
+
+    // the voxels this thread updates, loaded now and consumed at the end
+    float vold[BP_TZ];
+#pragma unroll
+    for (int k = 0; k < BP_TZ; k++) vold[k] = vslice[(size_t)min(z0 + k, vol_z - 1) * vol_x + xc];
 
     float acc[BP_TZ];
     int cnt[BP_TZ];
@@ -196,56 +200,69 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
     int full_cnt = 0;
     const double fx = (double)xc - vol_x * 0.5;
     const double fz0 = (double)z0 - vol_z * 0.5;
-    const float fxt0 = (float)((double)x0 - vol_x * 0.5);
-    const float fxt1 = (float)((double)(x0 + BP_TX - 1) - vol_x * 0.5);
-    const float fzt1 = (float)(fz0 + (BP_TZ - 1));
     const S* base = sino + (size_t)y * rows_per_slice * stride;
+    const int2* tile_tab = tiles + (size_t)zt * n_xtiles + xt;
+    const size_t tilt_step = (size_t)n_ztiles * n_xtiles;
 
     for (int c0 = 0; c0 < n_list; c0 += BP_CHUNK) {
         const int nc = min(BP_CHUNK, n_list - c0);
-        // ---- per-tilt window of this tile (one thread per tilt) ----
-        if (tid < nc) {
-            const TiltRow tr = list[c0 + tid];
-            const int cl = cls[((size_t)tr.tilt * n_ztiles + zt) * n_xtiles + xt];
-            const float2 sc = cs[tr.tilt];
-            const float umin = 0.5f * proj + fminf(sc.x * fxt0, sc.x * fxt1) +
-                               fminf(sc.y * (float)fz0, sc.y * fzt1);
-            s_w0[tid] = (int)floorf(umin) - 3;
-            s_row[tid] = tr.row;
-            s_tilt[tid] = tr.tilt;
-            s_cls[tid] = cl;
-            s_cs[tid] = sc;
-        }
-        __syncthreads();
-        // ---- stage the row segments: independent loads, one pass ----
-        for (int idx = tid; idx < nc * BP_WIN; idx += BP_TX) {
-            const int j = idx / BP_WIN;
-            const int i = idx - j * BP_WIN;
-            float2 pr = make_float2(0.0f, 0.0f);
-            const int g = s_w0[j] + i;
-            if (s_cls[j] != 0 && g >= -1 && g <= proj - 1) {
-                const S* row = base + (size_t)s_row[j] * stride;
-                const int ga = max(g, 0);
-                const int gb = min(ga + 1, proj - 1);
-                float p = ld<S>(row + ga), q = ld<S>(row + gb);
-                if (p <= thr) p = kSentinel;
-                if (q <= thr) q = kSentinel;
-                if (g == -1) pr = make_float2(p - (q - p), q - p);
-                else if (g == proj - 1) pr = make_float2(p, 0.0f);
-                else pr = make_float2(p, q - p);
+        // ---- stage the row segments.  Pair i of tilt j is (q(g), q(g+1) - q(g))
+        // at g = w0 + i, where q is the row extended linearly beyond its ends:
+        // q(g) = p[ga] + (g - ga) (p[ga+1] - p[ga]), ga = clamp(g, 0, P-2).
+        // Inside [0, P-2] that is the plain pair; at g = -1 it extrapolates, so
+        // a coordinate that FP32 rounding puts just below 0 still interpolates
+        // the first sample.  Pairs further out are only read by voxels the
+        // exact intervals exclude.  Four tilts' loads are issued before any
+        // pair is formed. ----
+        for (int h = 0; h < nc; h += 4) {
+            float pv[4][2], rv[4][2];
+            int gv[4][2];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const TiltRow tr = list[c0 + min(h + q, nc - 1)];
+                const int w0 = tile_tab[(size_t)tr.tilt * tilt_step].x;
+                const S* row = base + (size_t)tr.row * stride;
+#pragma unroll
+                for (int m = 0; m < 2; m++) {
+                    const int g = w0 + tid + m * BP_TX;
+                    const int ga = min(max(g, 0), proj - 2);
+                    if (m == 0 || tid < BP_WIN - BP_TX) {
+                        pv[q][m] = ld<S>(row + (unsigned)ga);
+                        rv[q][m] = ld<S>(row + (unsigned)ga + 1u);
+                    }
+                    gv[q][m] = g - ga;
+                }
             }
-            pairs[j][i] = pr;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (h + q >= nc) break;
+#pragma unroll
+                for (int m = 0; m < 2; m++) {
+                    if (m == 1 && tid >= BP_WIN - BP_TX) break;
+                    float p = pv[q][m], r = rv[q][m];
+                    if constexpr (!std::is_same<S, float>::value) {
+                        // narrow storage cannot hold -1e30: widen its empty marks
+                        const float thr = empty_threshold<S>();
+                        if (p <= thr) p = kSentinel;
+                        if (r <= thr) r = kSentinel;
+                    }
+                    const float d = r - p;
+                    pairs[h + q][tid + m * BP_TX] = make_float2(fmaf((float)gv[q][m], d, p), d);
+                }
+            }
         }
         __syncthreads();
         // ---- accumulate ----
         for (int j = 0; j < nc; j++) {
-            const int cl = s_cls[j];
-            if (cl == 0) continue;
-            const float c = s_cs[j].x, s = s_cs[j].y;
-            const double kk = fz0 * (double)s + proj / 2.0 - (double)s_w0[j];
+            const TiltRow tr = list[c0 + j];
+            const int2 tw = tile_tab[(size_t)tr.tilt * tilt_step];   // {w0, class}
+            if (tw.y == 0) continue;
+            const float2 sc = cs[tr.tilt];
+            const float c = sc.x, s = sc.y;
+            const double kk = fz0 * (double)s + proj / 2.0 - (double)tw.x;
             const float u0 = (float)(fx * (double)c + kk);
             const float2* pj = pairs[j];
-            if (cl == 1 && !sent) {
+            if (tw.y == 1 && !sent) {
                 full_cnt += 1;
 #pragma unroll
                 for (int k = 0; k < BP_TZ; k++) {
@@ -256,17 +273,16 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
                     acc[k] += fmaf(f, pr.y, pr.x);
                 }
             } else {
-                const int tilt = s_tilt[j];
+                const int2* lh = lohi + (size_t)tr.tilt * vol_z;
 #pragma unroll
                 for (int k = 0; k < BP_TZ; k++) {
-                    const int z = min(z0 + k, vol_z - 1);
-                    const int2 r = lohi[(size_t)tilt * vol_z + z];
+                    const int2 r = lh[min(z0 + k, vol_z - 1)];
                     float u = fmaf((float)k, s, u0);
                     int i = (int)u;
                     float f = __builtin_amdgcn_fractf(u);
                     float2 pr = pj[i];
                     float v = fmaf(f, pr.y, pr.x);
-                    bool in = (x >= r.x) && (x <= r.y) && !(v <= kEmpty);
+                    bool in = (x >= r.x) & (x <= r.y) & !(v <= kEmpty);
                     acc[k] += in ? v : 0.0f;
                     cnt[k] += in ? 1 : 0;
                 }
@@ -274,21 +290,19 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
         }
         __syncthreads();
     }
+    // ---- v = relax * sum / count + v, positivity, sentinel (GBackProj.cu:50-59) ----
     if (x >= vol_x) return;
     bool wrote_sent = false;
 #pragma unroll
     for (int k = 0; k < BP_TZ; k++) {
         const int z = z0 + k;
-        if (z >= vol_z) break;
         const int count = full_cnt + cnt[k];
-        if (count <= 0) continue;
-        const size_t vi = ((size_t)y * vol_z + z) * vol_x + x;
-        float v = relax * acc[k] / (float)count + vol[vi];
-        if (v > kEmpty) {
-            vol[vi] = positivity ? fmaxf(v, 0.0f) : v;
-        } else {
-            vol[vi] = kSentinel;
-            wrote_sent = true;
+        float v = relax * acc[k] * __builtin_amdgcn_rcpf((float)count) + vold[k];
+        const bool empty = !(v > kEmpty);
+        v = empty ? kSentinel : (positivity ? fmaxf(v, 0.0f) : v);
+        if (z < vol_z && count > 0) {
+            vslice[(size_t)z * vol_x + x] = v;
+            wrote_sent |= empty;
         }
     }
     if (wrote_sent) atomicOr(vflag + y, 1);
@@ -664,7 +678,7 @@ static void launch_bp_t(const Plan& p, const void* sino, int rows, const TiltRow
     int nxt = (p.vol_x + BP_TX - 1) / BP_TX, nzt = (p.vol_z + BP_TZ - 1) / BP_TZ;
     dim3 grid(nxt, nzt, p.vol_y);
     hipLaunchKernelGGL(k_back_proj<S>, grid, dim3(BP_TX), 0, s, (const S*)sino, rows, p.sino_stride,
-                       p.proj, list, n_list, p.d_cs, p.d_bp_lohi, p.d_bp_class, nzt, nxt, relax,
+                       p.proj, list, n_list, p.d_cs, p.d_bp_lohi, p.d_bp_tiles, nzt, nxt, relax,
                        p.positivity, check_sent ? p.d_sflag : nullptr, p.d_vol, p.vol_x, p.vol_z,
                        p.d_vflag);
 }

@@ -75,3 +75,27 @@ def test_prestage_matches_oracle(gpu_available, method):
     err = rel_l2(gpu, ref)
     print("rel_l2", err)
     assert err <= TOL32
+
+
+@pytest.mark.parametrize("case", ["sart_xzy", "sart_il_xzy", "wbp_xzy"])
+def test_golden_fixture(gpu_available, case):
+    import os
+    g = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                  "series_small.npz")))
+    print(f"case golden {case}", flush=True)
+    raw = g["raw"]
+    n, py, px = raw.shape
+    opts = {
+        "sart_xzy": dict(method=L.METHOD_SART, sart_iters=3, sart_subsets=3),
+        "sart_il_xzy": dict(method=L.METHOD_SART, sart_iters=3, sart_subsets=3, relax=0.3,
+                            subset_order=L.SUBSETS_INTERLEAVED, positivity=False),
+        "wbp_xzy": dict(method=L.METHOD_WBP),
+    }[case]
+    method = opts.pop("method")
+    plan = Plan(px, py, g["tilts"], int(g["vol_z"]), method, tilt_axis=g["axes"],
+                shifts=g["shifts"], fill_seed=int(g["seed"]), **opts)
+    gpu = plan.execute_host(raw)
+    err = rel_l2(gpu, g[case])
+    print("rel_l2", err)
+    assert gpu.shape == g[case].shape
+    assert err <= TOL32
