@@ -10,8 +10,8 @@ reference operators (``recon``) and the synthetic-data generator
 """
 from . import _lib
 from ._lib import AtrError
-from .recon import (DeviceArray, DoSartRecon, DoWbpRecon, Plan, TomoSart, TomoWbp,
+from .recon import (Batch, DeviceArray, DoSartRecon, DoWbpRecon, Plan, TomoSart, TomoWbp,
                     device_count, synchronize)
 
-__all__ = ["AtrError", "DeviceArray", "DoSartRecon", "DoWbpRecon", "Plan", "TomoSart",
+__all__ = ["AtrError", "Batch", "DeviceArray", "DoSartRecon", "DoWbpRecon", "Plan", "TomoSart",
            "TomoWbp", "device_count", "synchronize", "_lib"]

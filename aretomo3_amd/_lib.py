@@ -32,7 +32,10 @@ EXPORTS = [
     "atr_memcpy_d2h", "atr_memset", "atr_copy_kernel", "atr_synchronize", "atr_stream_create",
     "atr_stream_destroy", "atr_stream_synchronize", "atr_mem_info", "atr_event_create",
     "atr_event_destroy", "atr_event_record", "atr_event_elapsed_ms",
+    "atr_batch_create", "atr_batch_execute", "atr_batch_set_profiling", "atr_batch_kernel_stats",
+    "atr_batch_reset_stats", "atr_batch_destroy",
 ]
+BATCH_GRAPH = 1
 
 
 class SeriesDesc(ctypes.Structure):
@@ -116,6 +119,12 @@ def lib():
         "atr_event_destroy": (i, [vp]),
         "atr_event_record": (i, [vp, vp]),
         "atr_event_elapsed_ms": (i, [vp, vp, ctypes.POINTER(ctypes.c_float)]),
+        "atr_batch_create": (i, [ctypes.POINTER(vp), i, i, ctypes.POINTER(vp)]),
+        "atr_batch_execute": (i, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), vp]),
+        "atr_batch_set_profiling": (i, [vp, i]),
+        "atr_batch_kernel_stats": (i, [vp, i, pi, ctypes.POINTER(ctypes.c_double)]),
+        "atr_batch_reset_stats": (i, [vp]),
+        "atr_batch_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

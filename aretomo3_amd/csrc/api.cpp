@@ -16,10 +16,6 @@
 
 #include "atr_internal.h"
 
-struct atr_plan {
-    atr::Plan p;
-};
-
 namespace atr {
 
 static thread_local std::string g_err;
@@ -250,7 +246,17 @@ static atr_status plan_build(Plan& p)
         if ((st = upload(&p.d_fp_len, g.fp_len, B)) != ATR_OK) return st;
         if ((st = upload(&p.d_fp_geo, g.fp_geo, B)) != ATR_OK) return st;
         if ((st = upload(&p.d_fp_desc, g.fp_desc, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_pairs, fp_pair_table(p.proj), B)) != ATR_OK) return st;
+        {
+            std::vector<int> sched;
+            p.subset_pair_off.clear();
+            for (int sb = 0; sb < p.n_subsets; sb++) {
+                p.subset_pair_off.push_back((int)sched.size());
+                const int f0 = p.subset_fp_off[sb], f1 = p.subset_fp_off[sb + 1];
+                fp_pair_schedule(g, p.subset_fp.data() + f0, f1 - f0, p.proj, fp_group(p.proj),
+                                 sched);
+            }
+            if ((st = upload(&p.d_fp_pairs, sched, B)) != ATR_OK) return st;
+        }
         std::vector<TiltRow> lists(p.all_tilts);
         lists.insert(lists.end(), p.subset_tilts.begin(), p.subset_tilts.end());
         if ((st = upload(&p.d_lists, lists, B)) != ATR_OK) return st;
@@ -263,6 +269,21 @@ static atr_status plan_build(Plan& p)
         if ((st = dalloc(&p.d_diff, (size_t)p.vol_y * p.max_subset * p.sino_stride * sb, B)) != ATR_OK) return st;
         if ((st = dalloc((void**)&p.d_vflag, sizeof(int) * p.vol_y, B)) != ATR_OK) return st;
         if ((st = dalloc((void**)&p.d_vstats, sizeof(double) * 2 * p.vol_y, B)) != ATR_OK) return st;
+        // kernel arguments of every SART step, read by the kernels from device
+        // memory: [0] initial back-projection, [1 + s] subset s
+        const int n_all = (int)p.all_tilts.size();
+        std::vector<BpArgs> bpa;
+        bpa.push_back(bp_args(p, p.d_sino, p.n, p.d_lists, n_all, 1.0f, true));
+        std::vector<FpArgs> fpa;
+        for (int s = 0; s < p.n_subsets; s++) {
+            const int b0 = p.subset_off[s], b1 = p.subset_off[s + 1];
+            const int f0 = p.subset_fp_off[s], f1 = p.subset_fp_off[s + 1];
+            bpa.push_back(bp_args(p, p.d_diff, p.max_subset, p.d_lists + n_all + b0, b1 - b0,
+                                  p.relax, false));
+            fpa.push_back(fp_args(p, p.d_fp_lists + f0, f1 - f0, p.d_fp_pairs + p.subset_pair_off[s]));
+        }
+        if ((st = upload(&p.d_bp_args, bpa, B)) != ATR_OK) return st;
+        if ((st = upload(&p.d_fp_args, fpa, B)) != ATR_OK) return st;
     } else {
         build_wbp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.wbp_lohi);
         if ((st = upload(&p.d_wbp_lohi, g.wbp_lohi, B)) != ATR_OK) return st;
@@ -296,7 +317,7 @@ static void plan_free(Plan& p)
                     p.d_fill_aln, p.d_fill_vol, p.d_filter, p.d_tw_proj, p.d_tw_x2, p.d_tw_z2,
                     p.d_tw_z, p.d_tw_x, p.d_axis_cs, p.d_shifts, p.d_excl, p.d_aligned,
                     p.d_row_stats, p.d_tilt_stats, p.d_sino, p.d_diff, p.d_vol, p.d_sflag,
-                    p.d_vflag, p.d_vstats, p.d_wbp_t, p.d_wbp_u};
+                    p.d_vflag, p.d_vstats, p.d_wbp_t, p.d_wbp_u, p.d_bp_args, p.d_fp_args};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& e : p.ev_pool) (void)hipEventDestroy(e);
@@ -356,10 +377,10 @@ static void timed(Plan& p, int kernel, hipStream_t s, F&& fn)
 }
 
 // ---- execution -----------------------------------------------------------
-static atr_status plan_execute(Plan& p, const float* in, float* out, int accumulate, hipStream_t s)
+// SART before the iterations: pre-stage, weighting into the slice-major
+// sinogram, the volume state (zero, or the caller's SLICE volume).
+atr_status sart_prepare(Plan& p, const float* in, float* out, int accumulate, hipStream_t s)
 {
-    if (in == nullptr || out == nullptr) return fail(ATR_ERR_INVALID_ARG, "NULL buffer");
-    (void)hipGetLastError();
     if (p.prestage) {
         timed(p, K_PRESTAGE, s, [&] {
             launch_prestage(p, in, s);
@@ -367,33 +388,46 @@ static atr_status plan_execute(Plan& p, const float* in, float* out, int accumul
         });
     }
     launch_zero_int(p.d_sflag, p.vol_y, s);
+    launch_zero_int(p.d_vflag, p.vol_y, s);
+    timed(p, K_WEIGHT, s, [&] { launch_weight_sart(p, in, s); });
+    if (accumulate && p.layout == ATR_LAYOUT_SLICE) launch_load_volume(p, out, s);
+    else launch_zero(p.d_vol, (size_t)p.vol_x * p.vol_y * p.vol_z, s);
+    return ATR_OK;
+}
+
+// SART after the iterations: empty fill (CTomoSart.cpp:86-89) + layout.
+void sart_finish(Plan& p, float* out, hipStream_t s)
+{
+    timed(p, K_OUTPUT, s, [&] {
+        launch_vol_stats(p, s);
+        launch_output(p, out, s);
+    });
+}
+
+static atr_status plan_execute(Plan& p, const float* in, float* out, int accumulate, hipStream_t s)
+{
+    if (in == nullptr || out == nullptr) return fail(ATR_ERR_INVALID_ARG, "NULL buffer");
+    (void)hipGetLastError();
     if (p.method == ATR_METHOD_SART) {
-        launch_zero_int(p.d_vflag, p.vol_y, s);
-        timed(p, K_WEIGHT, s, [&] { launch_weight_sart(p, in, s); });
-        if (accumulate && p.layout == ATR_LAYOUT_SLICE) launch_load_volume(p, out, s);
-        else launch_zero(p.d_vol, (size_t)p.vol_x * p.vol_y * p.vol_z, s);
-        const TiltRow* d_all = p.d_lists;
-        const int n_all = (int)p.all_tilts.size();
+        sart_prepare(p, in, out, accumulate, s);
         // initial back-projection of every projection, relax 1 (CTomoSart.cpp:68-69)
-        if (n_all > 0)
-            timed(p, K_BP, s, [&] { launch_bp(p, p.d_sino, p.n, d_all, n_all, 1.0f, true, s); });
+        if (!p.all_tilts.empty()) timed(p, K_BP, s, [&] { launch_bp(p, 0, s); });
         for (int it = 0; it < p.iters; it++) {
             for (int sb = 0; sb < p.n_subsets; sb++) {
-                int f0 = p.subset_fp_off[sb], f1 = p.subset_fp_off[sb + 1];
-                int b0 = p.subset_off[sb], b1 = p.subset_off[sb + 1];
-                timed(p, K_FP, s, [&] { launch_fp_diff(p, p.d_fp_lists + f0, f1 - f0, s); });
-                if (b1 > b0)
-                    timed(p, K_BP, s, [&] {
-                        launch_bp(p, p.d_diff, p.max_subset, d_all + n_all + b0, b1 - b0,
-                                  p.relax, false, s);
-                    });
+                timed(p, K_FP, s, [&] { launch_fp_diff(p, sb, s); });
+                if (p.subset_off[sb + 1] > p.subset_off[sb])
+                    timed(p, K_BP, s, [&] { launch_bp(p, 1 + sb, s); });
             }
         }
-        timed(p, K_OUTPUT, s, [&] {
-            launch_vol_stats(p, s);
-            launch_output(p, out, s);
-        });
+        sart_finish(p, out, s);
     } else {
+        if (p.prestage) {
+            timed(p, K_PRESTAGE, s, [&] {
+                launch_prestage(p, in, s);
+                launch_tilt_stats(p, s);
+            });
+        }
+        launch_zero_int(p.d_sflag, p.vol_y, s);
         timed(p, K_WEIGHT, s, [&] { launch_filter_wbp(p, in, s); });
         for (int y0 = 0; y0 < p.vol_y; y0 += p.wbp_slab) {
             int ny = std::min(p.wbp_slab, p.vol_y - y0);

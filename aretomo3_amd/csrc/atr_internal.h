@@ -69,6 +69,9 @@ struct Geometry {
     int fp_bz = 0, fp_nbands = 0;      // z rows per band
 };
 
+struct BpArgs;
+struct FpArgs;
+
 struct Plan {
     atr_series_desc desc{};
     // sizes
@@ -90,6 +93,7 @@ struct Plan {
     std::vector<TiltRow> subset_tilts;// BP lists (excluded tilts removed)
     std::vector<int> subset_fp;       // FP tilt lists (all tilts of the subset)
     std::vector<int> subset_fp_off;
+    std::vector<int> subset_pair_off; // [n_subsets] offsets into d_fp_pairs
     std::vector<TiltRow> all_tilts;   // initial BP list
     // geometry
     Geometry geo;
@@ -103,7 +107,7 @@ struct Plan {
     int* d_fp_len = nullptr;
     float4* d_fp_geo = nullptr;
     int4* d_fp_desc = nullptr;
-    int* d_fp_pairs = nullptr;
+    int* d_fp_pairs = nullptr;        // per subset: FP wave schedules (fp_pair_schedule)
     TiltRow* d_lists = nullptr;       // all_tilts then every subset's BP list
     int* d_fp_lists = nullptr;        // every subset's FP list
     int* d_fill_aln = nullptr;        // [aln_x*aln_y]
@@ -127,6 +131,8 @@ struct Plan {
     int* d_sflag = nullptr;           // [vol_y] sinogram slice holds sentinels
     int* d_vflag = nullptr;           // [vol_y] volume slice holds sentinels
     double* d_vstats = nullptr;       // [vol_y][2] fill statistics of volume slices
+    BpArgs* d_bp_args = nullptr;      // SART step arguments: initial BP, then per subset
+    FpArgs* d_fp_args = nullptr;      // per subset
     float2* d_wbp_t = nullptr;        // [slab][2VolZ][VolX/2+1]
     float2* d_wbp_u = nullptr;        // [slab][VolZ][VolX/2+1]
     int wbp_slab = 0;
@@ -141,6 +147,56 @@ struct Plan {
     double total_ms[K_COUNT] = {0};
     hipStream_t stream = nullptr;
 };
+
+// ---- kernel arguments ----------------------------------------------------
+// SART back-projection of one list of tilts into one series' volume.
+struct BpArgs {
+    const void* sino;          // [VolY][rows_per_slice][stride] storage type
+    int rows_per_slice, stride, proj;
+    const TiltRow* list;
+    int n_list;
+    const float2* cs;
+    const int2* lohi;          // [N][VolZ]
+    const int2* tiles;         // [N][ztiles][xtiles] {window start, class}
+    int n_ztiles, n_xtiles;
+    float relax;
+    int positivity;
+    const int* sflag;          // NULL: the sinogram holds no empty samples
+    float* vol;
+    int vol_x, vol_z;
+    int* vflag;
+};
+// Forward projection + difference of one list of tilts of one series.
+struct FpArgs {
+    const float* vol;
+    int vol_x, vol_z;
+    const int* vflag;
+    const int* fp_list;
+    int n_list, group;
+    const int* sched;          // fp_pair_schedule blocks, one per group
+    const float2* cs;
+    const int2* fp_range;
+    const float4* geo;
+    const int4* fp_desc;
+    int nb, bz, proj;
+    const void* sino;
+    int n, stride;
+    void* diff;
+    int diff_rows;
+};
+// Entries of the batched (work-queue) launches; item0 = first queue index.
+struct BpItem {
+    BpArgs a;
+    int item0, n_items_y;      // items of one slice = ztiles * xtiles
+};
+struct FpItem {
+    FpArgs a;
+    int item0, n_groups;       // items of one slice = tilt groups
+};
+
+// ---- SART stages shared by single-plan and batched execution (api.cpp) ----
+atr_status sart_prepare(Plan& p, const float* in, float* out, int accumulate, hipStream_t s);
+void sart_finish(Plan& p, float* out, hipStream_t s);
 
 // ---- error handling ------------------------------------------------------
 void set_error(const std::string& msg);
@@ -163,7 +219,9 @@ void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                      int bz, Geometry& g);
 int fp_band_rows(int vol_x);
-std::vector<int> fp_pair_table(int proj);
+int fp_group(int proj);
+void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, int group,
+                      std::vector<int>& tab);
 void split_items(int n_items, int n_splits, std::vector<int>& starts, std::vector<int>& sizes);
 void fill_table(uint64_t seed, int n, std::vector<int>& table);
 void aligned_size(int raw_x, int raw_y, float axis_deg, int* ax, int* ay);
@@ -180,9 +238,18 @@ void launch_filter_wbp(const Plan& p, const float* in, hipStream_t s);
 void launch_zero(float* ptr, size_t n, hipStream_t s);
 void launch_zero_int(int* ptr, int n, hipStream_t s);
 int wbp_col_block(int vol_z);
-void launch_bp(const Plan& p, const void* sino, int rows_per_slice, const TiltRow* list, int n_list,
-               float relax, bool check_sent, hipStream_t s);
-void launch_fp_diff(const Plan& p, const int* fp_list, int n_list, hipStream_t s);
+BpArgs bp_args(const Plan& p, const void* sino, int rows, const TiltRow* list, int n_list,
+               float relax, bool check_sent);
+FpArgs fp_args(const Plan& p, const int* fp_list, int n_list, const int* sched);
+size_t fp_lds_bytes(int bz, int vol_x);
+// step 0: initial back-projection, 1 + s: subset s (arguments in p.d_bp_args)
+void launch_bp(const Plan& p, int step, hipStream_t s);
+void launch_bp_batch(int storage, const BpItem* items, int n_items, int total, int* counter,
+                     int grid, hipStream_t s);
+void launch_fp_batch(int storage, const FpItem* items, int n_items, int total, int* counters,
+                     size_t shm, int grid, hipStream_t s);
+int batch_blocks_per_cu(int storage, bool fp, size_t fp_shm);
+void launch_fp_diff(const Plan& p, int subset, hipStream_t s);
 void launch_vol_stats(const Plan& p, hipStream_t s);
 void launch_output(const Plan& p, float* out, hipStream_t s);
 void launch_wbp_bp_rowfft(const Plan& p, int y0, int ny, hipStream_t s);
@@ -192,3 +259,7 @@ void launch_load_volume(const Plan& p, const float* in, hipStream_t s);
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 }  // namespace atr
+
+struct atr_plan {
+    atr::Plan p;
+};

@@ -345,11 +345,118 @@ int fp_band_rows(int vol_x)
 
 // (tilt-in-group, tile) of every pair index p = wave + 16 i of the forward
 // projector: j << 16 | tile.
-std::vector<int> fp_pair_table(int proj)
+// Tilts per forward-projection workgroup: all FP_WAVES * FP_MAXP pair slots
+// filled with 64-ray tiles.
+int fp_group(int proj)
 {
     const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
-    std::vector<int> tab(FP_WAVES * FP_MAXP);
-    for (int p = 0; p < (int)tab.size(); p++) tab[p] = ((p / ntile) << 16) | (p % ntile);
-    return tab;
+    return std::max(1, FP_WAVES * FP_MAXP / ntile);
+}
+
+// Assignment of the (tilt, 64-ray tile) pairs of one forward-projection
+// launch to the FP_WAVES waves of a workgroup (FP_MAXP slots each).  Waves
+// meet at a barrier after every z-band, so what a band costs is the largest
+// per-wave load in it; the schedule minimises the sum over bands of that
+// maximum: greedy placement by decreasing cost, then pairwise exchanges.
+// Cost of a pair in a band = the samples its descriptor covers, masked
+// windows weighted 1.4, plus a fixed 8 per non-empty descriptor.
+// tab gets one block of FP_WAVES * FP_MAXP entries per group of `group`
+// tilts: entry [w * FP_MAXP + i] = (local tilt << 16) | tile, or -1.
+void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, int group,
+                      std::vector<int>& tab)
+{
+    const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
+    const int nb = g.fp_nbands;
+    for (int j0 = 0; j0 < n_list; j0 += group) {
+        const int ng = std::min(group, n_list - j0);
+        const int n_items = ng * ntile;
+        std::vector<double> cost((size_t)n_items * nb);
+        std::vector<double> total(n_items, 0.0);
+        for (int it = 0; it < n_items; it++) {
+            const int t = list[j0 + it / ntile], tile = it % ntile;
+            for (int b = 0; b < nb; b++) {
+                const int4 d = g.fp_desc[((size_t)t * ntile + tile) * nb + b];
+                double c = d.y > 0 ? d.y * (d.w ? 1.0 : 1.4) + 8.0 : 0.0;
+                cost[(size_t)it * nb + b] = c;
+                total[it] += c;
+            }
+        }
+        std::vector<int> order(n_items);
+        for (int i = 0; i < n_items; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return total[a] > total[b]; });
+        std::vector<double> load((size_t)FP_WAVES * nb, 0.0);
+        std::vector<int> owner(n_items, -1), fill(FP_WAVES, 0);
+        auto band_max = [&](int b) {
+            double m = 0.0;
+            for (int w = 0; w < FP_WAVES; w++) m = std::max(m, load[(size_t)w * nb + b]);
+            return m;
+        };
+        std::vector<double> bmax(nb, 0.0);
+        for (int it : order) {
+            int best = -1;
+            double best_obj = 0.0, best_tot = 0.0;
+            for (int w = 0; w < FP_WAVES; w++) {
+                if (fill[w] >= FP_MAXP) continue;
+                double obj = 0.0, tot = 0.0;
+                for (int b = 0; b < nb; b++) {
+                    const double l = load[(size_t)w * nb + b] + cost[(size_t)it * nb + b];
+                    obj += std::max(bmax[b], l);
+                    tot += l;
+                }
+                if (best < 0 || obj < best_obj - 1e-9 || (obj < best_obj + 1e-9 && tot < best_tot)) {
+                    best = w;
+                    best_obj = obj;
+                    best_tot = tot;
+                }
+            }
+            owner[it] = best;
+            fill[best]++;
+            for (int b = 0; b < nb; b++) {
+                load[(size_t)best * nb + b] += cost[(size_t)it * nb + b];
+                bmax[b] = std::max(bmax[b], load[(size_t)best * nb + b]);
+            }
+        }
+        // pairwise exchanges between waves while they lower the objective
+        auto objective = [&]() {
+            double o = 0.0;
+            for (int b = 0; b < nb; b++) o += band_max(b);
+            return o;
+        };
+        double cur = objective();
+        for (int pass = 0; pass < 4; pass++) {
+            bool improved = false;
+            for (int a = 0; a < n_items; a++)
+                for (int c = a + 1; c < n_items; c++) {
+                    const int u = owner[a], v = owner[c];
+                    if (u == v) continue;
+                    for (int b = 0; b < nb; b++) {
+                        const double d = cost[(size_t)c * nb + b] - cost[(size_t)a * nb + b];
+                        load[(size_t)u * nb + b] += d;
+                        load[(size_t)v * nb + b] -= d;
+                    }
+                    const double o = objective();
+                    if (o < cur - 1e-9) {
+                        cur = o;
+                        std::swap(owner[a], owner[c]);
+                        improved = true;
+                    } else {
+                        for (int b = 0; b < nb; b++) {
+                            const double d = cost[(size_t)c * nb + b] - cost[(size_t)a * nb + b];
+                            load[(size_t)u * nb + b] -= d;
+                            load[(size_t)v * nb + b] += d;
+                        }
+                    }
+                }
+            if (!improved) break;
+        }
+        const size_t base = tab.size();
+        tab.resize(base + FP_WAVES * FP_MAXP, -1);
+        std::vector<int> slot(FP_WAVES, 0);
+        for (int it = 0; it < n_items; it++) {
+            const int w = owner[it];
+            tab[base + w * FP_MAXP + slot[w]++] = ((it / ntile) << 16) | (it % ntile);
+        }
+    }
 }
 }  // namespace atr

@@ -14,6 +14,10 @@
 
 namespace atr {
 
+// A value every lane holds alike, moved to an SGPR so that the compiler
+// treats what is derived from it as wave-uniform.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // ------------------------------------------------------------------------
 // Pre-stage: Correct/GCorrPatchShift.cu:58-100 (global shift + tilt-axis
 // rotation, nearest neighbour, -1e30 outside the raw image), geometry in
@@ -168,22 +172,32 @@ __global__ __launch_bounds__(256) void k_weight(const float* __restrict__ in, si
 // order, as the reference.  Everything per tilt (list entry, cos/sin, the
 // tile's window start and class) is wave-uniform and read with scalar loads.
 // ------------------------------------------------------------------------
+// One tile: voxels x0 .. x0+BP_TX-1, z0 .. z0+BP_TZ-1 of slice y.  Ends
+// after a barrier, with the LDS free for the next tile.
 template <typename S>
-__global__ __launch_bounds__(BP_TX) void k_back_proj(
-    const S* __restrict__ sino, int rows_per_slice, int stride, int proj,
-    const TiltRow* __restrict__ list, int n_list, const float2* __restrict__ cs,
-    const int2* __restrict__ lohi, const int2* __restrict__ tiles, int n_ztiles, int n_xtiles,
-    float relax, int positivity, const int* __restrict__ sflag, float* __restrict__ vol,
-    int vol_x, int vol_z, int* __restrict__ vflag)
+__device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[BP_CHUNK][BP_WIN],
+                                        int xt, int zt, int y)
 {
-    __shared__ float2 pairs[BP_CHUNK][BP_WIN];
+    const gptr<const S> sino = (gptr<const S>)A->sino;
+    const int rows_per_slice = A->rows_per_slice, stride = A->stride, proj = A->proj;
+    const cptr<TiltRow> list = as_cptr(A->list);
+    const int n_list = A->n_list;
+    const cptr<float2> cs = as_cptr(A->cs);
+    const cptr<int2> lohi = as_cptr(A->lohi);
+    const cptr<int2> tiles = as_cptr(A->tiles);
+    const int n_ztiles = A->n_ztiles, n_xtiles = A->n_xtiles;
+    const float relax = A->relax;
+    const int positivity = A->positivity;
+    const cptr<int> sflag = as_cptr(A->sflag);
+    const gptr<float> vol = as_gptr(A->vol);
+    const int vol_x = A->vol_x, vol_z = A->vol_z;
+    int* vflag = A->vflag;
     const int tid = threadIdx.x;
-    const int xt = blockIdx.x, zt = blockIdx.y, y = blockIdx.z;
     const int x0 = xt * BP_TX, z0 = zt * BP_TZ;
     const int x = x0 + tid;
     const int xc = min(x, vol_x - 1);
     const bool sent = sflag != nullptr && sflag[y] != 0;
-    float* vslice = vol + (size_t)y * vol_z * vol_x;
+    const gptr<float> vslice = vol + (size_t)y * vol_z * vol_x;
 
     // the voxels this thread updates, loaded now and consumed at the end
     float vold[BP_TZ];
@@ -200,8 +214,8 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
     int full_cnt = 0;
     const double fx = (double)xc - vol_x * 0.5;
     const double fz0 = (double)z0 - vol_z * 0.5;
-    const S* base = sino + (size_t)y * rows_per_slice * stride;
-    const int2* tile_tab = tiles + (size_t)zt * n_xtiles + xt;
+    const gptr<const S> base = sino + (size_t)y * rows_per_slice * stride;
+    const cptr<int2> tile_tab = tiles + (size_t)zt * n_xtiles + xt;
     const size_t tilt_step = (size_t)n_ztiles * n_xtiles;
 
     for (int c0 = 0; c0 < n_list; c0 += BP_CHUNK) {
@@ -219,16 +233,16 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
             int gv[4][2];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const TiltRow tr = list[c0 + min(h + q, nc - 1)];
+                const TiltRow tr = ldc(list + c0 + min(h + q, nc - 1));
                 const int w0 = tile_tab[(size_t)tr.tilt * tilt_step].x;
-                const S* row = base + (size_t)tr.row * stride;
+                const gptr<const S> row = base + (size_t)tr.row * stride;
 #pragma unroll
                 for (int m = 0; m < 2; m++) {
                     const int g = w0 + tid + m * BP_TX;
                     const int ga = min(max(g, 0), proj - 2);
                     if (m == 0 || tid < BP_WIN - BP_TX) {
-                        pv[q][m] = ld<S>(row + (unsigned)ga);
-                        rv[q][m] = ld<S>(row + (unsigned)ga + 1u);
+                        pv[q][m] = ldg(row + (unsigned)ga);
+                        rv[q][m] = ldg(row + (unsigned)ga + 1u);
                     }
                     gv[q][m] = g - ga;
                 }
@@ -254,10 +268,10 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
         __syncthreads();
         // ---- accumulate ----
         for (int j = 0; j < nc; j++) {
-            const TiltRow tr = list[c0 + j];
-            const int2 tw = tile_tab[(size_t)tr.tilt * tilt_step];   // {w0, class}
+            const TiltRow tr = ldc(list + c0 + j);
+            const int2 tw = ldc(tile_tab + (size_t)tr.tilt * tilt_step);   // {w0, class}
             if (tw.y == 0) continue;
-            const float2 sc = cs[tr.tilt];
+            const float2 sc = ldc(cs + tr.tilt);
             const float c = sc.x, s = sc.y;
             const double kk = fz0 * (double)s + proj / 2.0 - (double)tw.x;
             const float u0 = (float)(fx * (double)c + kk);
@@ -273,10 +287,10 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
                     acc[k] += fmaf(f, pr.y, pr.x);
                 }
             } else {
-                const int2* lh = lohi + (size_t)tr.tilt * vol_z;
+                const cptr<int2> lh = lohi + (size_t)tr.tilt * vol_z;
 #pragma unroll
                 for (int k = 0; k < BP_TZ; k++) {
-                    const int2 r = lh[min(z0 + k, vol_z - 1)];
+                    const int2 r = ldc(lh + min(z0 + k, vol_z - 1));
                     float u = fmaf((float)k, s, u0);
                     int i = (int)u;
                     float f = __builtin_amdgcn_fractf(u);
@@ -291,7 +305,6 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
         __syncthreads();
     }
     // ---- v = relax * sum / count + v, positivity, sentinel (GBackProj.cu:50-59) ----
-    if (x >= vol_x) return;
     bool wrote_sent = false;
 #pragma unroll
     for (int k = 0; k < BP_TZ; k++) {
@@ -300,7 +313,7 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
         float v = relax * acc[k] * __builtin_amdgcn_rcpf((float)count) + vold[k];
         const bool empty = !(v > kEmpty);
         v = empty ? kSentinel : (positivity ? fmaxf(v, 0.0f) : v);
-        if (z < vol_z && count > 0) {
+        if (x < vol_x && z < vol_z && count > 0) {
             vslice[(size_t)z * vol_x + x] = v;
             wrote_sent |= empty;
         }
@@ -308,26 +321,63 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(
     if (wrote_sent) atomicOr(vflag + y, 1);
 }
 
+template <typename S>
+__global__ __launch_bounds__(BP_TX) void k_back_proj(const BpArgs* __restrict__ a)
+{
+    __shared__ float2 pairs[BP_CHUNK][BP_WIN];
+    bp_tile<S>(as_cptr(a), pairs, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Block-wide work queue: thread 0 takes the next item index, every thread
+// sees it.  Items past the end tell every block to leave.
+__device__ __forceinline__ int next_item(int* __restrict__ counter)
+{
+    __shared__ int s_item;
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = uniform(s_item);
+    __syncthreads();
+    return it;
+}
+
+// Back-projection of many series in one persistent launch: blocks take
+// (series, slice, z-tile, x-tile) items from a queue; entries are ordered by
+// their first item.
+template <typename S>
+__global__ __launch_bounds__(BP_TX) void k_back_proj_batch(const BpItem* __restrict__ e, int n_e,
+                                                           int total, int* __restrict__ counter)
+{
+    __shared__ float2 pairs[BP_CHUNK][BP_WIN];
+    const cptr<BpItem> E = as_cptr(e);
+    for (int it = next_item(counter); it < total; it = next_item(counter)) {
+        int s = 0;
+        while (s + 1 < n_e && E[s + 1].item0 <= it) s++;
+        const int local = it - E[s].item0;
+        const int per_y = E[s].n_items_y;
+        const int y = local / per_y, r = local - y * per_y;
+        const int zt = r / E[s].a.n_xtiles, xt = r - zt * E[s].a.n_xtiles;
+        bp_tile<S>(&E[s].a, pairs, xt, zt, y);
+    }
+}
+
 // ------------------------------------------------------------------------
 // Forward projection + difference (Recon/GForProj.cu:10-76 and
 // Recon/GDiffProj.cu:7-21 fused).
 //
 // One workgroup (FP_THREADS) per (slice, group of tilts).  The slice is
-// streamed through LDS in z-bands of bz cell rows; cell (z, x) holds the
-// bilinear coefficients of the unit square above it,
-//   F = (V[z][x], V[z][x+1]-V[z][x], V[z+1][x]-V[z][x], d(x+1)-d(x) of both rows),
-// so a sample is a + f b + wz (c + f d): one ds_read_b128 and three FMAs.
-// Cell 0 of a row is a guard for x in [-1, 0) (linear extrapolation), which
-// lets the cell index be a plain truncation of x + 1.  Wave w owns the
-// (tilt, 64-ray tile) pairs w, w+16, ..., one ray per lane and one FP32
-// accumulator per pair, so every wave carries the same mix of tilts and the
-// per-band barrier does not leave waves idle.  FP64 tables give, per (tilt,
-// band), the samples whose cell row lies in the band and, per (tilt, tile,
-// band), whether every ray of the tile uses all of them; only the other
-// blocks test each sample against the ray's FP64 interval.  Samples are
-// processed four at a time with their geometry and cells loaded before any
-// is used.  The difference raw - fwd (0 where either is empty) is written for
-// the back-projection.
+// streamed through LDS in z-bands of bz cell rows; band entry (z, x) holds the
+// interpolation pair (V[z][x], V[z][x+1]-V[z][x]), so a bilinear sample is two
+// ds_read_b64 (rows z and z+1) and three FMAs.  Entry 0 of a row is a guard for
+// x in [-1, 0) (linear extrapolation), which lets the entry index be a plain
+// truncation of x + 1.  The host assigns the group's (tilt, 64-ray tile)
+// pairs to the FP_WAVES waves (FP_MAXP slots each) so that the per-band loads
+// of the waves match (fp_pair_schedule); one ray per lane and one FP32
+// accumulator per pair.  FP64 tables give, per (tilt, tile, band), the
+// samples whose cell row lies in the band and whether every ray of the tile
+// uses all of them; only the other blocks test each sample against the ray's
+// FP64 interval.  Samples are processed four at a time with their geometry
+// and cells loaded before any is used.  The difference raw - fwd (0 where
+// either is empty) is written for the back-projection.
 // ------------------------------------------------------------------------
 // Geometry of four consecutive samples, fetched with one 64-byte scalar load.
 struct Geo4 {
@@ -356,12 +406,12 @@ __device__ __forceinline__ float fp_sample(const char* __restrict__ band, float4
 // tests each sample against the ray's FP64 interval rr = (first, count).
 template <bool MASKED, bool SENT>
 __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
-                                       const float4* __restrict__ G, int count, int k0,
+                                       cptr<float4> G, int count, int k0,
                                        float tf1, float xmax1, int2 rr, float& acc, int& cnt)
 {
     int m = 0;
     for (; m + 4 <= count; m += 4) {
-        const Geo4 g4 = *(const Geo4*)(G + m);
+        const Geo4 g4 = ldc((cptr<Geo4>)(G + m));
         float v[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -382,7 +432,7 @@ __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
         }
     }
     if (m < count) {
-        const Geo4 g4 = *(const Geo4*)(G + m);
+        const Geo4 g4 = ldc((cptr<Geo4>)(G + m));
         float v[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -402,29 +452,39 @@ __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
     }
 }
 
+// One (group of tilts, slice) work item.  Returns early, for the whole
+// block, when the slice's sentinel flag does not match SENT.  Ends after a
+// barrier, with the LDS free for the next item.
 template <typename S, bool SENT>
-__global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(
-    const float* __restrict__ vol, int vol_x, int vol_z, const int* __restrict__ vflag,
-    const int* __restrict__ fp_list, int n_list, int group, const int* __restrict__ pair_tab,
-    const float2* __restrict__ cs, const int2* __restrict__ fp_range,
-    const float4* __restrict__ geo, const int4* __restrict__ fp_desc, int nb, int bz, int proj,
-    const S* __restrict__ sino, int n, int stride, S* __restrict__ diff, int diff_rows)
+__device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ band, int gx, int y)
 {
-    extern __shared__ float2 band[];
-    const int y = blockIdx.y;
-    if ((vflag[y] != 0) != SENT) return;
+    const gptr<const float> vol = as_gptr(A->vol);
+    const int vol_x = A->vol_x, vol_z = A->vol_z;
+    const cptr<int> fp_list = as_cptr(A->fp_list);
+    const int group = A->group;
+    const cptr<int> sched = as_cptr(A->sched);
+    const cptr<float2> cs = as_cptr(A->cs);
+    const gptr<const int2> fp_range = as_gptr(A->fp_range);
+    const cptr<float4> geo = as_cptr(A->geo);
+    const cptr<int4> fp_desc = as_cptr(A->fp_desc);
+    const int nb = A->nb, bz = A->bz, proj = A->proj;
+    const gptr<const S> sino = (gptr<const S>)A->sino;
+    const int n = A->n, stride = A->stride;
+    const gptr<S> diff = (gptr<S>)A->diff;
+    const int diff_rows = A->diff_rows;
+    if ((as_cptr(A->vflag)[y] != 0) != SENT) return;
     const int tid = threadIdx.x, lane = tid & 63;
     // wave index made provably uniform so that all per-pair table reads are
     // scalar loads into SGPRs
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int j0 = blockIdx.x * group;
-    const int ng = min(group, n_list - j0);
+    const int j0 = gx * group;
     const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
-    const int npairs = ng * ntile;
+    // this wave's (local tilt << 16 | tile) slots, -1 = unused
+    const cptr<int> pair_tab = sched + (size_t)gx * (FP_WAVES * FP_MAXP) + wave * FP_MAXP;
     const int pitch = vol_x + 1;               // pairs per band row (guard + VolX)
     const int rowb = pitch * (int)sizeof(float2);
     const float xmax1 = (float)(vol_x - 1);    // clamp of x + 1 for masked samples
-    const float* V = vol + (size_t)y * vol_z * vol_x;
+    const gptr<const float> V = vol + (size_t)y * vol_z * vol_x;
     float acc[FP_MAXP], tf1[FP_MAXP];
     int cnt[FP_MAXP];
 #pragma unroll
@@ -432,9 +492,8 @@ __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(
         acc[i] = 0.0f;
         cnt[i] = 0;
         tf1[i] = 1.0f;
-        const int p = wave + FP_WAVES * i;
-        if (p < npairs) {
-            const int jt = pair_tab[p];
+        const int jt = pair_tab[i];
+        if (jt >= 0) {
             const int t = fp_list[j0 + (jt >> 16)];
             const int xd = min((jt & 0xffff) * FP_WAVE + lane, proj - 1);
             tf1[i] = (float)(((double)xd - 0.5 * proj) / (double)cs[t].x + vol_x * 0.5 + 1.0);
@@ -451,7 +510,7 @@ __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(
         for (int col = tid; col < pitch; col += FP_THREADS) {
             const int xi = col - 1;                      // -1 .. VolX-1
             const int xa = max(xi, 0), xb = min(xa + 1, vol_x - 1);
-            const float* base = V + (size_t)zb * vol_x;
+            const gptr<const float> base = V + (size_t)zb * vol_x;
 #pragma unroll
             for (int h = 0; h < FP_BZ_MAX + 1; h += FP_STAGE_ROWS) {
                 float va[FP_STAGE_ROWS], vb[FP_STAGE_ROWS];
@@ -475,14 +534,13 @@ __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < FP_MAXP; i++) {
-            const int p = wave + FP_WAVES * i;
-            if (p < npairs) {
-                const int jt = pair_tab[p];
+            const int jt = pair_tab[i];
+            if (jt >= 0) {
                 const int t = fp_list[j0 + (jt >> 16)];
                 const int tile = jt & 0xffff;
-                const int4 d = fp_desc[((size_t)t * ntile + tile) * nb + b];
+                const int4 d = ldc(fp_desc + ((size_t)t * ntile + tile) * nb + b);
                 if (d.y > 0) {
-                    const float4* G = geo + d.x;
+                    const cptr<float4> G = geo + d.x;
                     if (!SENT && d.w != 0) {
                         fp_run<false, false>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1,
                                              make_int2(0, 0), acc[i], cnt[i]);
@@ -491,7 +549,8 @@ __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(
                         // keep the interval load inside the band loop: hoisted,
                         // the intervals of all pairs would pin 2*FP_MAXP VGPRs
                         asm volatile("" : "+v"(xd));
-                        const int2 rr = xd < proj ? fp_range[(size_t)t * proj + xd] : make_int2(0, 0);
+                        int2 rr = make_int2(0, 0);
+                        if (xd < proj) rr = ldgo(fp_range + (size_t)t * proj + xd);
                         fp_run<true, SENT>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1, rr, acc[i],
                                            cnt[i]);
                     }
@@ -502,21 +561,46 @@ __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(
     }
 #pragma unroll
     for (int i = 0; i < FP_MAXP; i++) {
-        const int p = wave + FP_WAVES * i;
-        if (p < npairs) {
-            const int jt = pair_tab[p];
+        const int jt = pair_tab[i];
+        if (jt >= 0) {
             const int j = j0 + (jt >> 16);
             const int t = fp_list[j];
             const int xd = (jt & 0xffff) * FP_WAVE + lane;
             if (xd < proj) {
                 const int count = SENT ? cnt[i] : fp_range[(size_t)t * proj + xd].y;
                 const float fwd = count <= 0 ? kSentinel : acc[i] / (float)count;
-                float raw = ld<S>(sino + ((size_t)y * n + t) * stride + xd);
+                float raw = ldg(sino + ((size_t)y * n + t) * stride + xd);
                 if (raw <= empty_threshold<S>()) raw = kSentinel;
                 const float d = (fwd < kEmpty || raw < kEmpty) ? 0.0f : raw - fwd;
-                st<S>(diff + ((size_t)y * diff_rows + j) * stride + xd, d);
+                stg(diff + ((size_t)y * diff_rows + j) * stride + xd, d);
             }
         }
+    }
+}
+
+template <typename S, bool SENT>
+__global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(const FpArgs* __restrict__ a)
+{
+    extern __shared__ float2 band[];
+    fp_block<S, SENT>(as_cptr(a), band, blockIdx.x, blockIdx.y);
+}
+
+// Forward projection + difference of many series in one persistent launch:
+// blocks take (series, slice, tilt group) items from a queue.
+template <typename S, bool SENT>
+__global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff_batch(const FpItem* __restrict__ e,
+                                                                   int n_e, int total,
+                                                                   int* __restrict__ counter)
+{
+    extern __shared__ float2 band[];
+    const cptr<FpItem> E = as_cptr(e);
+    for (int it = next_item(counter); it < total; it = next_item(counter)) {
+        int s = 0;
+        while (s + 1 < n_e && E[s + 1].item0 <= it) s++;
+        const int local = it - E[s].item0;
+        const int ng = E[s].n_groups;
+        const int y = local / ng, gx = local - y * ng;
+        fp_block<S, SENT>(&E[s].a, band, gx, y);
     }
 }
 
@@ -671,57 +755,161 @@ void launch_weight_sart(const Plan& p, const float* in, hipStream_t s)
     else launch_weight_t<float>(p, in, s);
 }
 
-template <typename S>
-static void launch_bp_t(const Plan& p, const void* sino, int rows, const TiltRow* list, int n_list,
-                        float relax, bool check_sent, hipStream_t s)
+BpArgs bp_args(const Plan& p, const void* sino, int rows, const TiltRow* list, int n_list,
+               float relax, bool check_sent)
 {
-    int nxt = (p.vol_x + BP_TX - 1) / BP_TX, nzt = (p.vol_z + BP_TZ - 1) / BP_TZ;
+    BpArgs a{};
+    a.sino = sino;
+    a.rows_per_slice = rows;
+    a.stride = p.sino_stride;
+    a.proj = p.proj;
+    a.list = list;
+    a.n_list = n_list;
+    a.cs = p.d_cs;
+    a.lohi = p.d_bp_lohi;
+    a.tiles = p.d_bp_tiles;
+    a.n_ztiles = (p.vol_z + BP_TZ - 1) / BP_TZ;
+    a.n_xtiles = (p.vol_x + BP_TX - 1) / BP_TX;
+    a.relax = relax;
+    a.positivity = p.positivity;
+    a.sflag = check_sent ? p.d_sflag : nullptr;
+    a.vol = p.d_vol;
+    a.vol_x = p.vol_x;
+    a.vol_z = p.vol_z;
+    a.vflag = p.d_vflag;
+    return a;
+}
+
+void launch_bp(const Plan& p, int step, hipStream_t s)
+{
+    const int nxt = (p.vol_x + BP_TX - 1) / BP_TX, nzt = (p.vol_z + BP_TZ - 1) / BP_TZ;
     dim3 grid(nxt, nzt, p.vol_y);
-    hipLaunchKernelGGL(k_back_proj<S>, grid, dim3(BP_TX), 0, s, (const S*)sino, rows, p.sino_stride,
-                       p.proj, list, n_list, p.d_cs, p.d_bp_lohi, p.d_bp_tiles, nzt, nxt, relax,
-                       p.positivity, check_sent ? p.d_sflag : nullptr, p.d_vol, p.vol_x, p.vol_z,
-                       p.d_vflag);
+    const BpArgs* a = p.d_bp_args + step;
+    if (p.storage == ATR_STORAGE_F16) hipLaunchKernelGGL(k_back_proj<_Float16>, grid, dim3(BP_TX), 0, s, a);
+    else if (p.storage == ATR_STORAGE_BF16) hipLaunchKernelGGL(k_back_proj<bf16_t>, grid, dim3(BP_TX), 0, s, a);
+    else hipLaunchKernelGGL(k_back_proj<float>, grid, dim3(BP_TX), 0, s, a);
 }
 
-void launch_bp(const Plan& p, const void* sino, int rows, const TiltRow* list, int n_list,
-               float relax, bool check_sent, hipStream_t s)
+void launch_bp_batch(int storage, const BpItem* items, int n_items, int total, int* counter,
+                     int grid, hipStream_t s)
 {
-    if (p.storage == ATR_STORAGE_F16) launch_bp_t<_Float16>(p, sino, rows, list, n_list, relax, check_sent, s);
-    else if (p.storage == ATR_STORAGE_BF16) launch_bp_t<bf16_t>(p, sino, rows, list, n_list, relax, check_sent, s);
-    else launch_bp_t<float>(p, sino, rows, list, n_list, relax, check_sent, s);
+    if (storage == ATR_STORAGE_F16)
+        hipLaunchKernelGGL(k_back_proj_batch<_Float16>, dim3(grid), dim3(BP_TX), 0, s, items, n_items, total, counter);
+    else if (storage == ATR_STORAGE_BF16)
+        hipLaunchKernelGGL(k_back_proj_batch<bf16_t>, dim3(grid), dim3(BP_TX), 0, s, items, n_items, total, counter);
+    else
+        hipLaunchKernelGGL(k_back_proj_batch<float>, dim3(grid), dim3(BP_TX), 0, s, items, n_items, total, counter);
 }
 
-template <typename S, bool SENT>
-static void launch_fp_one(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
+FpArgs fp_args(const Plan& p, const int* fp_list, int n_list, const int* sched)
 {
-    const int ntile = (p.proj + FP_WAVE - 1) / FP_WAVE;
-    const int group = std::max(1, FP_WAVES * FP_MAXP / ntile);
-    dim3 grid((n_list + group - 1) / group, p.vol_y);
-    size_t shm = (size_t)(p.geo.fp_bz + 1) * (p.vol_x + 1) * sizeof(float2);
-    static bool attr_set = false;  // per instantiation: allow > 64 KiB of dynamic LDS
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_for_proj_diff<S, SENT>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, FP_LDS_BUDGET);
-        attr_set = true;
+    FpArgs a{};
+    a.vol = p.d_vol;
+    a.vol_x = p.vol_x;
+    a.vol_z = p.vol_z;
+    a.vflag = p.d_vflag;
+    a.fp_list = fp_list;
+    a.n_list = n_list;
+    a.group = fp_group(p.proj);
+    a.sched = sched;
+    a.cs = p.d_cs;
+    a.fp_range = p.d_fp_range;
+    a.geo = p.d_fp_geo;
+    a.fp_desc = p.d_fp_desc;
+    a.nb = p.geo.fp_nbands;
+    a.bz = p.geo.fp_bz;
+    a.proj = p.proj;
+    a.sino = p.d_sino;
+    a.n = p.n;
+    a.stride = p.sino_stride;
+    a.diff = p.d_diff;
+    a.diff_rows = p.max_subset;
+    return a;
+}
+
+size_t fp_lds_bytes(int bz, int vol_x)
+{
+    return (size_t)(bz + 1) * (vol_x + 1) * sizeof(float2);
+}
+
+// Kernels with more than 64 KiB of dynamic LDS must say so once.
+template <typename K>
+static void allow_big_lds(K kernel, bool& done)
+{
+    if (!done) {
+        (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  FP_LDS_BUDGET);
+        done = true;
     }
-    hipLaunchKernelGGL((k_for_proj_diff<S, SENT>), grid, dim3(FP_THREADS), shm, s, p.d_vol,
-                       p.vol_x, p.vol_z, p.d_vflag, fp_list, n_list, group, p.d_fp_pairs, p.d_cs,
-                       p.d_fp_range, p.d_fp_geo, p.d_fp_desc, p.geo.fp_nbands, p.geo.fp_bz, p.proj,
-                       (const S*)p.d_sino, p.n, p.sino_stride, (S*)p.d_diff, p.max_subset);
 }
 
 template <typename S>
-static void launch_fp_t(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
+static void launch_fp_t(const Plan& p, int subset, hipStream_t s)
 {
-    launch_fp_one<S, false>(p, fp_list, n_list, s);
-    launch_fp_one<S, true>(p, fp_list, n_list, s);
+    static bool done0 = false, done1 = false;
+    allow_big_lds(k_for_proj_diff<S, false>, done0);
+    allow_big_lds(k_for_proj_diff<S, true>, done1);
+    const int n_list = p.subset_fp_off[subset + 1] - p.subset_fp_off[subset];
+    const int group = fp_group(p.proj);
+    dim3 grid((n_list + group - 1) / group, p.vol_y);
+    const size_t shm = fp_lds_bytes(p.geo.fp_bz, p.vol_x);
+    const FpArgs* a = p.d_fp_args + subset;
+    hipLaunchKernelGGL((k_for_proj_diff<S, false>), grid, dim3(FP_THREADS), shm, s, a);
+    hipLaunchKernelGGL((k_for_proj_diff<S, true>), grid, dim3(FP_THREADS), shm, s, a);
 }
 
-void launch_fp_diff(const Plan& p, const int* fp_list, int n_list, hipStream_t s)
+void launch_fp_diff(const Plan& p, int subset, hipStream_t s)
 {
-    if (p.storage == ATR_STORAGE_F16) launch_fp_t<_Float16>(p, fp_list, n_list, s);
-    else if (p.storage == ATR_STORAGE_BF16) launch_fp_t<bf16_t>(p, fp_list, n_list, s);
-    else launch_fp_t<float>(p, fp_list, n_list, s);
+    if (p.storage == ATR_STORAGE_F16) launch_fp_t<_Float16>(p, subset, s);
+    else if (p.storage == ATR_STORAGE_BF16) launch_fp_t<bf16_t>(p, subset, s);
+    else launch_fp_t<float>(p, subset, s);
+}
+
+template <typename S>
+static void fp_batch_attrs()
+{
+    static bool done0 = false, done1 = false;
+    allow_big_lds(k_for_proj_diff_batch<S, false>, done0);
+    allow_big_lds(k_for_proj_diff_batch<S, true>, done1);
+}
+
+template <typename S>
+static void launch_fp_batch_t(const FpItem* items, int n_items, int total, int* counters,
+                              size_t shm, int grid, hipStream_t s)
+{
+    fp_batch_attrs<S>();
+    hipLaunchKernelGGL((k_for_proj_diff_batch<S, false>), dim3(grid), dim3(FP_THREADS), shm, s,
+                       items, n_items, total, counters);
+    hipLaunchKernelGGL((k_for_proj_diff_batch<S, true>), dim3(grid), dim3(FP_THREADS), shm, s,
+                       items, n_items, total, counters + 1);
+}
+
+void launch_fp_batch(int storage, const FpItem* items, int n_items, int total, int* counters,
+                     size_t shm, int grid, hipStream_t s)
+{
+    if (storage == ATR_STORAGE_F16) launch_fp_batch_t<_Float16>(items, n_items, total, counters, shm, grid, s);
+    else if (storage == ATR_STORAGE_BF16) launch_fp_batch_t<bf16_t>(items, n_items, total, counters, shm, grid, s);
+    else launch_fp_batch_t<float>(items, n_items, total, counters, shm, grid, s);
+}
+
+// Resident blocks per CU of the batched kernels (the persistent grid size).
+template <typename S>
+static int batch_blocks_t(size_t fp_shm, bool fp)
+{
+    fp_batch_attrs<S>();
+    int nb = 0;
+    hipError_t e = fp ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                            &nb, k_for_proj_diff_batch<S, false>, FP_THREADS, fp_shm)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_back_proj_batch<S>,
+                                                                     BP_TX, 0);
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+int batch_blocks_per_cu(int storage, bool fp, size_t fp_shm)
+{
+    if (storage == ATR_STORAGE_F16) return batch_blocks_t<_Float16>(fp_shm, fp);
+    if (storage == ATR_STORAGE_BF16) return batch_blocks_t<bf16_t>(fp_shm, fp);
+    return batch_blocks_t<float>(fp_shm, fp);
 }
 
 void launch_zero(float* ptr, size_t n, hipStream_t s)

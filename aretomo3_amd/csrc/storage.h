@@ -41,6 +41,45 @@ __device__ __forceinline__ void st<bf16_t>(bf16_t* p, float v)
     p->bits = (uint16_t)(b >> 16);
 }
 
+// Address-space-qualified pointers.  Tables a whole wave reads alike are
+// reached through constant-space pointers (scalar loads); per-lane data
+// through global-space pointers (global_load / global_store, never flat).
+#define ATR_CONST_AS __attribute__((address_space(4)))
+#define ATR_GLOBAL_AS __attribute__((address_space(1)))
+template <typename T>
+using cptr = const ATR_CONST_AS T*;
+template <typename T>
+using gptr = ATR_GLOBAL_AS T*;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_cptr(const T* p) { return (cptr<T>)p; }
+template <typename T>
+__device__ __forceinline__ gptr<T> as_gptr(T* p) { return (gptr<T>)p; }
+
+// Whole-object loads through qualified pointers (the address space is
+// recovered by the compiler from the cast's source).
+template <typename T>
+__device__ __forceinline__ T ldc(cptr<T> p) { return *(const T*)p; }
+template <typename T>
+__device__ __forceinline__ T ldgo(const ATR_GLOBAL_AS T* p) { return *(const T*)p; }
+
+__device__ __forceinline__ float ldg(const ATR_GLOBAL_AS float* p) { return *p; }
+__device__ __forceinline__ float ldg(const ATR_GLOBAL_AS _Float16* p) { return (float)*p; }
+__device__ __forceinline__ float ldg(const ATR_GLOBAL_AS bf16_t* p)
+{
+    return __uint_as_float((uint32_t)p->bits << 16);
+}
+__device__ __forceinline__ void stg(ATR_GLOBAL_AS float* p, float v) { *p = v; }
+__device__ __forceinline__ void stg(ATR_GLOBAL_AS _Float16* p, float v)
+{
+    *p = (_Float16)fmaxf(v, -65504.0f);
+}
+__device__ __forceinline__ void stg(ATR_GLOBAL_AS bf16_t* p, float v)
+{
+    uint32_t b = __float_as_uint(v);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    p->bits = (uint16_t)(b >> 16);
+}
+
 // Threshold below which a stored value counts as the reference's "empty"
 // (<= -1e10).  FP16 cannot hold -1e10, so its sentinel is the most negative
 // finite half; real data never gets there after weighting by cos/VolZ.

@@ -193,6 +193,51 @@ class Plan:
             pass
 
 
+class Batch:
+    """Several SART plans reconstructed together (atr_batch_*): every
+    back-projection / forward-projection step of all series is one persistent
+    launch; graph=True captures the iteration sequence in a HIP graph once and
+    replays it.  The plans must outlive the batch."""
+
+    def __init__(self, plans, graph=False):
+        self.plans = list(plans)
+        arr = (ctypes.c_void_p * len(self.plans))(*[p.handle for p in self.plans])
+        h = ctypes.c_void_p()
+        check(lib().atr_batch_create(arr, len(self.plans), L.BATCH_GRAPH if graph else 0,
+                                     ctypes.byref(h)))
+        self.handle = h.value
+
+    def execute(self, projs, vols, stream=None):
+        """projs / vols: one device buffer (DeviceArray or pointer) per plan."""
+        def ptr(x):
+            return x.ptr if isinstance(x, DeviceArray) else x
+        ins = (ctypes.c_void_p * len(projs))(*[ptr(x) for x in projs])
+        outs = (ctypes.c_void_p * len(vols))(*[ptr(x) for x in vols])
+        check(lib().atr_batch_execute(self.handle, ins, outs, stream))
+
+    def set_profiling(self, on=True):
+        check(lib().atr_batch_set_profiling(self.handle, 1 if on else 0))
+
+    def kernel_stats(self, kernel):
+        n, ms = ctypes.c_int(), ctypes.c_double()
+        check(lib().atr_batch_kernel_stats(self.handle, kernel, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def reset_stats(self):
+        check(lib().atr_batch_reset_stats(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().atr_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # --------------------------------------------------------------------------
 # per-slice operators (CTomoWbp / CTomoSart)
 # --------------------------------------------------------------------------

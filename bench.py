@@ -1,26 +1,34 @@
 #!/usr/bin/env python3
-"""Benchmark of the SART reconstruction hot path on one MI355X.
+"""Benchmark of the reconstruction hot path on one MI355X.
 
-Workload (the headline SART series): seeded phantom of 200 random ellipsoids
-(semi-axes 4-40 voxels, density 0.5-1.0), 61 tilts -60..+60 deg step 2,
-1024x1024 raw projections with tilt axis 85 deg and per-tilt shifts uniform
-+-3 px (supplied as alignment, so the pre-stage runs), Gaussian noise of 10%
-of the mean; volume 1024 x 1024 x 256; 20 SART iterations, 5 interleaved
-subsets, relaxation 0.3, positivity off; FP32 projection storage.
+Default workload (the headline SART series, --config sart1024): seeded phantom
+of 200 random ellipsoids (semi-axes 4-40 voxels, density 0.5-1.0), 61 tilts
+-60..+60 deg step 2, 1024x1024 raw projections with tilt axis 85 deg and
+per-tilt shifts uniform +-3 px (supplied as alignment, so the pre-stage runs),
+Gaussian noise of 10% of the mean; volume 1024 x 1024 x 256; 20 SART
+iterations, 5 interleaved subsets, relaxation 0.3, positivity off; FP32
+projection storage.
 
-A step is one whole reconstruction of that series with the raw projections
+A step is one whole reconstruction of the series with the raw projections
 already resident in HBM: pre-stage correction + fill, weighting, initial
 back-projection and 20 x 5 forward/difference/back-projection passes, and the
 volume written out.  The metric counts output-resolution voxel-tilt updates:
-VolX*VolY*VolZ*N per pass, 1 + 2*iterations passes.
+VolX*VolY*VolZ*N per pass; SART has 1 + 2*iterations passes, WBP one.
+
+Other configurations (--config): wbp512 (WBP, 512^2 x 41 tilts step 3, volume
+512x512x128, tilt axis 0, noise 5%), sart2048-f16 / sart2048-bf16 and
+wbp2048-f16 / wbp2048-bf16 (2048^2 x 61 tilts, volume 2048x2048x512, 2000
+ellipsoids of semi-axes 4-80, noise 10%, 10 SART iterations, projections
+stored in FP16 / BF16 with FP32 accumulation), small (a reduced SART case for
+quick checks; not a benchmark).
 
 Prints one JSON line.  Options: --steps K --warmup W --gpus N (only 1),
---no-cpu-baseline, --cpu-slices S, --small (a reduced configuration for
-quick checks; not the benchmark).
+--config NAME, --no-cpu-baseline, --cpu-slices S, --copy-probe MIB.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -31,14 +39,43 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gvoxel-tilt updates/s at ≤2e-3 rel-L2 vs FP64; p50/p99 tomogram latency"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = "profiles/hbm_traffic_sart_1024x61x256.md"
+
+SART_BENCH = dict(method="sart", relax=0.3, order="interleaved", positivity=False)
+CONFIGS = {
+    "sart1024": dict(SART_BENCH, px=1024, vol_z=256, step=2.0, n_ell=200, amax=40.0, axis=85.0,
+                     shift=3.0, noise=0.10, iters=20, subsets=5, storage="f32", seed=20261203,
+                     cpu_slices=32),
+    "wbp512": dict(method="wbp", px=512, vol_z=128, step=3.0, n_ell=200, amax=40.0, axis=0.0,
+                   shift=0.0, noise=0.05, storage="f32", seed=20261202, cpu_slices=512),
+    "sart2048-f16": dict(SART_BENCH, px=2048, vol_z=512, step=2.0, n_ell=2000, amax=80.0,
+                         axis=0.0, shift=0.0, noise=0.10, iters=10, subsets=5, storage="f16",
+                         seed=20261204, cpu_slices=16),
+    "sart2048-bf16": dict(SART_BENCH, px=2048, vol_z=512, step=2.0, n_ell=2000, amax=80.0,
+                          axis=0.0, shift=0.0, noise=0.10, iters=10, subsets=5, storage="bf16",
+                          seed=20261204, cpu_slices=16),
+    "wbp2048-f16": dict(method="wbp", px=2048, vol_z=512, step=2.0, n_ell=2000, amax=80.0,
+                        axis=0.0, shift=0.0, noise=0.10, storage="f16", seed=20261204,
+                        cpu_slices=16),
+    "wbp2048-bf16": dict(method="wbp", px=2048, vol_z=512, step=2.0, n_ell=2000, amax=80.0,
+                         axis=0.0, shift=0.0, noise=0.10, storage="bf16", seed=20261204,
+                         cpu_slices=16),
+    "small": dict(SART_BENCH, px=256, vol_z=64, step=2.0, n_ell=60, amax=20.0, axis=85.0,
+                  shift=3.0, noise=0.10, iters=4, subsets=5, storage="f32", seed=20261204,
+                  cpu_slices=8),
+}
 
 
-def workload(small):
-    if small:
-        return dict(px=256, py=256, vol_z=64, step=2.0, n_ell=60, amax=20.0, iters=4,
-                    subsets=5, seed=20261204)
-    return dict(px=1024, py=1024, vol_z=256, step=2.0, n_ell=200, amax=40.0, iters=20,
-                subsets=5, seed=20261203)
+def measured_traffic():
+    """HBM bytes per k_back_proj launch of the headline SART series from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by the
+    copy-kernel calibration), or None."""
+    try:
+        text = open(os.path.join(ROOT, TRAFFIC_FILE)).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch k_back_proj: ([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
 
 
 def copy_probe(mib):
@@ -55,14 +92,32 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
 
 
+def describe(name, w, vol):
+    vx, vy, vz = vol
+    n = int(round(120.0 / w["step"])) + 1
+    geo = ("tilt axis %g deg + shifts +-%g px" % (w["axis"], w["shift"]) if w["shift"] > 0
+           else "tilt axis %g deg" % w["axis"])
+    if w["method"] == "sart":
+        alg = ("SART %d iterations x %d %s subsets, relax %g, positivity %s" %
+               (w["iters"], w["subsets"], w["order"], w["relax"], "on" if w["positivity"] else "off"))
+    else:
+        alg = "WBP (ramp + Hamming, 2x-grid back-projection, Fourier crop)"
+    return ("%s: %dx%d raw projections, %d tilts -60..60 step %g, %s, %d ellipsoids, "
+            "noise %g%%, volume %dx%dx%d, %s, %s projection storage" %
+            (name, w["px"], w["px"], n, w["step"], geo, w["n_ell"], 100 * w["noise"], vx, vy, vz,
+             alg, w["storage"].upper()))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="sart1024", choices=sorted(CONFIGS))
+    ap.add_argument("--small", action="store_true", help="same as --config small")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-slices", type=int, default=32)
-    ap.add_argument("--small", action="store_true")
+    ap.add_argument("--cpu-slices", type=int, default=0,
+                    help="y-slices the CPU baseline reconstructs (0 = the config's default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -71,28 +126,38 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the reconstruction shards over y-slabs with no exchange; see DESIGN.md)",
+              "(the reconstruction would shard over y-slabs with no exchange between GPUs)",
               file=sys.stderr)
         sys.exit(2)
+    name = "small" if args.small else args.config
+    w = CONFIGS[name]
 
     from aretomo3_amd import DeviceArray, Plan, synchronize
     from aretomo3_amd import _lib as L
     from aretomo3_amd import phantom
 
-    w = workload(args.small)
+    sart = w["method"] == "sart"
+    storage = {"f32": L.STORAGE_F32, "f16": L.STORAGE_F16, "bf16": L.STORAGE_BF16}[w["storage"]]
     tilts = phantom.tilt_range(-60.0, 60.0, w["step"])
     n = len(tilts)
     t0 = time.time()
-    raw, shifts = phantom.tilt_series(w["seed"], w["px"], w["py"], tilts, w["vol_z"],
+    raw, shifts = phantom.tilt_series(w["seed"], w["px"], w["px"], tilts, w["vol_z"],
                                       n_ellipsoids=w["n_ell"], axis_min=4.0,
-                                      axis_max=w["amax"], tilt_axis=85.0, max_shift=3.0,
-                                      noise_frac=0.10)
-    t_gen = time.time() - t0
-    print(f"[bench] generated {raw.shape} in {t_gen:.1f}s", file=sys.stderr, flush=True)
-    axes = np.full(n, 85.0, np.float32)
-    plan = Plan(w["px"], w["py"], tilts, w["vol_z"], L.METHOD_SART, tilt_axis=axes,
-                shifts=shifts, sart_iters=w["iters"], sart_subsets=w["subsets"], relax=0.3,
-                subset_order=L.SUBSETS_INTERLEAVED, positivity=False, fill_seed=w["seed"])
+                                      axis_max=w["amax"], tilt_axis=w["axis"],
+                                      max_shift=w["shift"], noise_frac=w["noise"],
+                                      n_threads=16)
+    print(f"[bench] {name}: generated {raw.shape} in {time.time() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    axes = np.full(n, w["axis"], np.float32)
+    if sart:
+        plan = Plan(w["px"], w["px"], tilts, w["vol_z"], L.METHOD_SART, tilt_axis=axes,
+                    shifts=shifts, sart_iters=w["iters"], sart_subsets=w["subsets"],
+                    relax=w["relax"], positivity=w["positivity"], storage=storage,
+                    subset_order=(L.SUBSETS_INTERLEAVED if w["order"] == "interleaved"
+                                  else L.SUBSETS_CONTIGUOUS), fill_seed=w["seed"])
+    else:
+        plan = Plan(w["px"], w["px"], tilts, w["vol_z"], L.METHOD_WBP, tilt_axis=axes,
+                    shifts=shifts, storage=storage, fill_seed=w["seed"])
     vol_x, vol_y, vol_z = plan.vol_x, plan.vol_y, plan.vol_z
     d_raw = DeviceArray.from_numpy(raw)
     d_vol = DeviceArray(4 * vol_x * vol_y * vol_z)
@@ -112,30 +177,39 @@ def main():
     wall = time.perf_counter() - t0
     plan.set_profiling(False)
     ms_step = 1e3 * wall / args.steps
+    print(f"[bench] {args.steps} steps in {wall:.2f}s", file=sys.stderr, flush=True)
 
-    passes = 1 + 2 * w["iters"]
-    updates = float(vol_x) * vol_y * vol_z * n * passes
-    value = updates / (wall / args.steps) / 1e9
-
-    # roofline of the SART back-projection (algorithmic bytes per launch:
-    # 8 B per voxel read-modify-write + 4 B per projection sample read)
-    bp_launches, bp_ms = plan.kernel_stats(L.K_BP)
-    fp_launches, fp_ms = plan.kernel_stats(L.K_FP)
-    per_step_bp = 1 + w["iters"] * w["subsets"]
+    passes = 1 + 2 * w["iters"] if sart else 1
     vox = float(vol_x) * vol_y * vol_z
-    proj_samples_step = float(n + w["iters"] * n) * plan.vol_x * vol_y
-    bytes_step = per_step_bp * 8.0 * vox + 4.0 * proj_samples_step
-    bytes_launch = bytes_step / per_step_bp
-    avg_ms = bp_ms / max(bp_launches, 1)
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    kernels = {}
-    for name, k in (("bp", L.K_BP), ("fp_diff", L.K_FP), ("weight", L.K_WEIGHT),
-                    ("prestage", L.K_PRESTAGE), ("output", L.K_OUTPUT)):
-        nl, ms = plan.kernel_stats(k)
-        kernels[name] = {"launches": nl, "ms_per_step": round(ms / args.steps, 3)}
+    updates = vox * n * passes
+    value = updates / (wall / args.steps) / 1e9
+    s_p = 4.0 if w["storage"] == "f32" else 2.0
 
-    # accuracy: GPU volume vs the FP64 oracle on sampled slices
-    vol = d_vol.to_numpy((vol_y, vol_z, vol_x))
+    kernels = {}
+    for kname, k in (("bp", L.K_BP), ("fp_diff", L.K_FP), ("weight", L.K_WEIGHT),
+                     ("prestage", L.K_PRESTAGE), ("output", L.K_OUTPUT),
+                     ("wbp_bp_rowfft", L.K_WBP_BP), ("wbp_colfft", L.K_WBP_COL),
+                     ("wbp_rowc2r", L.K_WBP_ROW)):
+        nl, ms = plan.kernel_stats(k)
+        if nl:
+            kernels[kname] = {"launches": nl, "ms_per_step": round(ms / args.steps, 3)}
+    if sart:
+        # SART back-projection, algorithmic bytes per launch: 8 B per voxel
+        # (read + write) + s_p per sinogram sample read, averaged over the
+        # 1 + iterations x subsets launches of a reconstruction
+        per_step = 1 + w["iters"] * w["subsets"]
+        bytes_launch = (per_step * 8.0 * vox + s_p * (n + w["iters"] * n) * vol_x * vol_y) / per_step
+        launches, kms = plan.kernel_stats(L.K_BP)
+        rkernel = "k_back_proj (SART back-projection)"
+    else:
+        # WBP back-projection + row FFT, per launch (one y-slab): s_p per sinogram
+        # sample read + 16 B per output voxel (2x-grid row spectra written)
+        launches, kms = plan.kernel_stats(L.K_WBP_BP)
+        bytes_launch = (s_p * n * vol_x * vol_y + 16.0 * vox) / max(launches / args.steps, 1)
+        rkernel = "k_wbp_bp_rowfft (2x-grid back-projection + row FFT)"
+    avg_ms = kms / max(launches, 1)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic() if name == "sart1024" else None
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -150,21 +224,20 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded ellipsoid phantom, analytic projections)",
         "config": {
-            "workload": ("SART one series: %dx%d raw projections, %d tilts -60..60 step %g, "
-                         "tilt axis 85 deg + shifts +-3 px, volume %dx%dx%d, %d iterations x "
-                         "%d interleaved subsets, relax 0.3, positivity off, FP32 storage")
-            % (w["px"], w["py"], n, w["step"], vol_x, vol_y, vol_z, w["iters"], w["subsets"]),
+            "workload": describe(name, w, (vol_x, vol_y, vol_z)),
             "updates_per_step": updates,
-            "reduced": bool(args.small),
+            "projection_storage": w["storage"],
         },
         "roofline": {
-            "kernel": "k_back_proj (SART back-projection)",
+            "kernel": rkernel,
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": (TRAFFIC_FILE + " (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per "
+                               "launch, copy-kernel calibrated)") if traffic else None,
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": round(avg_ms, 4),
         },
@@ -173,28 +246,33 @@ def main():
 
     if not args.no_cpu_baseline:
         import oracle
-        slices = np.linspace(0, vol_y - 1, min(args.cpu_slices, vol_y)).round().astype(np.int32)
+        vol = d_vol.to_numpy((vol_y, vol_z, vol_x))
+        n_sl = min(args.cpu_slices or w["cpu_slices"], vol_y)
+        slices = np.linspace(0, vol_y - 1, n_sl).round().astype(np.int32)
         threads = oracle.max_threads()
         t0 = time.perf_counter()
         aligned = oracle.prestage(raw, axes, shifts, 1, w["seed"])
         t_pre = time.perf_counter() - t0
         t0 = time.perf_counter()
-        ref = oracle.recon_slices(aligned, tilts, 1, vol_z, slices, iters=w["iters"],
-                                  n_subsets=w["subsets"], relax=0.3, order=1, positivity=0,
-                                  seed=w["seed"])
+        if sart:
+            ref = oracle.recon_slices(aligned, tilts, 1, vol_z, slices, iters=w["iters"],
+                                      n_subsets=w["subsets"], relax=w["relax"],
+                                      order=1 if w["order"] == "interleaved" else 0,
+                                      positivity=1 if w["positivity"] else 0, seed=w["seed"])
+        else:
+            ref = oracle.recon_slices(aligned, tilts, 0, vol_z, slices)
         t_cpu = time.perf_counter() - t0
         cpu_updates = float(vol_x) * len(slices) * vol_z * n * passes
-        from_vol = vol[slices]
-        num = np.linalg.norm((from_vol.astype(np.float64) - ref).ravel())
-        rel = float(num / np.linalg.norm(ref.ravel()))
+        got = vol[slices].astype(np.float64)
+        rel = float(np.linalg.norm((got - ref).ravel()) / np.linalg.norm(ref.ravel()))
         result["cpu_baseline"] = {
             "value": round(cpu_updates / t_cpu / 1e9, 4),
             "unit": "Gvoxel-tilt updates/s",
             "cores": threads,
             "kind": "port",
-            "sample": "FP64 OpenMP oracle, %d of %d y-slices of the same series (SART only; "
+            "sample": "FP64 OpenMP oracle, %d of %d y-slices of the same series (%s only; "
                       "pre-stage of all tilts took %.1fs, not counted), %.1fs" %
-                      (len(slices), vol_y, t_pre, t_cpu),
+                      (len(slices), vol_y, "SART" if sart else "WBP", t_pre, t_cpu),
         }
         result["rel_l2_vs_fp64"] = rel
     print(json.dumps(result))

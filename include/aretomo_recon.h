@@ -166,6 +166,28 @@ atr_status atr_phantom_tilt_series(uint64_t seed, int proj_x, int proj_y, int n_
                                    float max_shift, float noise_frac, float* shift_xy_out,
                                    float* proj_out, int n_threads);
 
+/* ---- batches: many SART series per launch ----------------------------------
+ * No single reference counterpart: the reference reconstructs one series at a
+ * time (CDoSartRecon::DoIt, AreTomo/Recon/CDoSartRecon.cpp:30-57, called per
+ * series from AreTomo/CAreTomoMain.cpp:809-848).  A batch runs every
+ * back-projection and forward-projection step of all its series as one
+ * persistent work-queue launch; with ATR_BATCH_GRAPH the iteration sequence
+ * is captured once into a HIP graph and replayed.  Each series gives the
+ * same volume as atr_execute on its plan alone.  Plans must be SART plans
+ * with one storage type, must outlive the batch, and must not be executed
+ * while the batch runs. */
+typedef struct atr_batch atr_batch;
+#define ATR_BATCH_GRAPH 1
+atr_status atr_batch_create(atr_plan* const* plans, int n_plans, int flags, atr_batch** batch);
+/* proj_dev[i] / vol_dev[i]: input and output of plans[i], as atr_execute. */
+atr_status atr_batch_execute(atr_batch* batch, const float* const* proj_dev,
+                             float* const* vol_dev, void* stream);
+atr_status atr_batch_set_profiling(atr_batch* batch, int enable);
+atr_status atr_batch_kernel_stats(const atr_batch* batch, int kernel, int* launches,
+                                  double* total_ms);
+atr_status atr_batch_reset_stats(atr_batch* batch);
+void atr_batch_destroy(atr_batch* batch);
+
 /* ---- device helpers (plumbing for hosts without a HIP binding) --------- */
 atr_status atr_device_count(int* count);
 atr_status atr_set_device(int device);

@@ -1,0 +1,172 @@
+"""Parity of the HIP path for the plan options beyond the reference default:
+narrow projection storage, output layouts, the per-slice operators, excluded
+tilts, odd widths, unfilled sentinels and a larger series.
+
+Tolerances (relative L2 against the FP64 oracle): 1e-5 for FP32 storage,
+2e-3 for FP16/BF16 storage.  Layout transforms are bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from aretomo3_amd import DeviceArray, Plan, TomoSart, TomoWbp, phantom
+from aretomo3_amd import _lib as L
+from helpers import rel_l2, small_series
+
+pytestmark = pytest.mark.gpu
+TOL32 = 1e-5
+TOL16 = 2e-3
+
+
+def _ref(proj, tilts, method, vol_z, **kw):
+    n, py, px = proj.shape
+    return oracle.recon_slices(proj.astype(np.float64), tilts, method, vol_z, list(range(py)),
+                               **kw)
+
+
+@pytest.mark.parametrize("storage", [L.STORAGE_F16, L.STORAGE_BF16])
+@pytest.mark.parametrize("method", [L.METHOD_SART, L.METHOD_WBP])
+def test_narrow_storage(gpu_available, storage, method):
+    tilts, proj, _ = small_series(px=64, py=6, vol_z=24)
+    print(f"case storage={storage} method={method}", flush=True)
+    plan = Plan(64, 6, tilts, 24, method, sart_iters=3, sart_subsets=3, storage=storage)
+    gpu = plan.execute_host(proj)
+    ref = _ref(proj, tilts, method, 24, iters=3, n_subsets=3)
+    err = rel_l2(gpu, ref)
+    print("rel_l2", err)
+    assert err <= TOL16
+
+
+@pytest.mark.parametrize("method", [L.METHOD_SART, L.METHOD_WBP])
+def test_layouts_are_exact_permutations(gpu_available, method):
+    tilts, proj, _ = small_series(px=48, py=5, vol_z=20)
+    print(f"case layouts method={method}", flush=True)
+    out = {}
+    for lay in (L.LAYOUT_XZY, L.LAYOUT_XYZ, L.LAYOUT_XYZ_NOFLIP, L.LAYOUT_SLICE):
+        plan = Plan(48, 5, tilts, 20, method, sart_iters=2, sart_subsets=2, layout=lay)
+        out[lay] = plan.execute_host(proj)
+        plan.close()
+    xzy = out[L.LAYOUT_XZY].astype(np.float64)
+    assert out[L.LAYOUT_XZY].shape == (5, 20, 48)
+    np.testing.assert_array_equal(out[L.LAYOUT_XYZ], oracle.flip_vol(xzy, 1).astype(np.float32))
+    np.testing.assert_array_equal(out[L.LAYOUT_XYZ_NOFLIP],
+                                  oracle.flip_vol(xzy, 0).astype(np.float32))
+    np.testing.assert_array_equal(out[L.LAYOUT_SLICE], out[L.LAYOUT_XZY][:, ::-1, :])
+
+
+def test_exclude_mask(gpu_available):
+    tilts, proj, _ = small_series(px=64, py=4, vol_z=24)
+    n = len(tilts)
+    mask = np.zeros(n, np.uint8)
+    mask[[1, 5, n - 2]] = 1
+    print("case exclude", flush=True)
+    for method in (L.METHOD_SART, L.METHOD_WBP):
+        plan = Plan(64, 4, tilts, 24, method, sart_iters=3, sart_subsets=4, exclude=mask)
+        gpu = plan.execute_host(proj)
+        ref = _ref(proj, tilts, method, 24, iters=3, n_subsets=4, no_proj=mask)
+        err = rel_l2(gpu, ref)
+        print("rel_l2", method, err)
+        assert err <= TOL32
+
+
+@pytest.mark.parametrize("px", [65, 81])
+def test_odd_width(gpu_available, px):
+    tilts, proj, _ = small_series(px=px, py=3, vol_z=18)
+    print(f"case odd px={px}", flush=True)
+    for method in (L.METHOD_SART, L.METHOD_WBP):
+        plan = Plan(px, 3, tilts, 18, method, sart_iters=2, sart_subsets=3)
+        gpu = plan.execute_host(proj)
+        assert gpu.shape == (3, 18, px // 2 * 2)
+        ref = _ref(proj, tilts, method, 18, iters=2, n_subsets=3)
+        err = rel_l2(gpu, ref)
+        print("rel_l2", method, err)
+        assert err <= TOL32
+
+
+def test_unfilled_sentinels(gpu_available):
+    # SART after a pre-stage without the random fill: the aligned images keep
+    # -1e30 where the rotation leaves the raw image, and the empty-sample rules
+    # of weighting, back-projection, forward projection and difference apply
+    # (WBP always runs on filled images: its ramp filter spreads any sentinel)
+    tilts, raw, shifts = small_series(px=48, py=40, vol_z=16, axis=85.0, shift=3.0)
+    axes = np.full(len(tilts), 85.0, np.float32)
+    print("case sentinels", flush=True)
+    aligned = oracle.prestage(raw, axes, shifts, 0, 0)
+    assert (aligned < -1e10).any()
+    for method in (L.METHOD_SART,):
+        plan = Plan(48, 40, tilts, 16, method, tilt_axis=axes, shifts=shifts, random_fill=False,
+                    sart_iters=2, sart_subsets=3, fill_seed=5)
+        gpu = plan.execute_host(raw)
+        ref = oracle.recon_slices(aligned, tilts, method, 16, list(range(aligned.shape[1])),
+                                  iters=2, n_subsets=3, seed=5)
+        assert np.isfinite(gpu).all()
+        err = rel_l2(gpu, ref)
+        print("rel_l2", method, err)
+        assert err <= TOL32
+
+
+def test_slice_operators(gpu_available):
+    # CTomoSart / CTomoWbp on a padded sinogram in device memory; TomoSart
+    # continues from the volume already in the buffer
+    tilts, proj, _ = small_series(px=64, py=1, vol_z=24)
+    n = len(tilts)
+    pad = (64 // 2 + 1) * 2
+    sino = np.zeros((n, pad), np.float32)
+    sino[:, :64] = proj[:, 0, :]
+    cs = oracle.cos_sin(tilts)
+    print("case slice operators", flush=True)
+    d_sino = DeviceArray.from_numpy(sino)
+    start = np.random.default_rng(3).random((24, 64)).astype(np.float32)
+    d_vol = DeviceArray.from_numpy(start)
+    op = TomoSart()
+    op.setup(64, 24, 3, 2, 64, tilts, 0, n)
+    op.do_it(d_sino, d_vol)
+    got = d_vol.to_numpy((24, 64))
+    table = oracle.fill_table(0 ^ 0x5A5A5A5A5A5A5A5A, 64 * 24)
+    ref, _ = oracle.tomo_sart(sino.astype(np.float64), 64, cs, start.astype(np.float64), 0, n,
+                              2, 3, fill_tbl=table)
+    err = rel_l2(got, ref)
+    print("rel_l2 sart", err)
+    assert err <= TOL32
+    d_vol2 = DeviceArray(4 * 24 * 64)
+    w = TomoWbp()
+    w.setup(64, 24, 64, tilts)
+    w.do_it(d_sino, d_vol2)
+    got = d_vol2.to_numpy((24, 64))
+    ref, _ = oracle.tomo_wbp(sino.astype(np.float64), 64, cs, 64, 24)
+    err = rel_l2(got, ref)
+    print("rel_l2 wbp", err)
+    assert err <= TOL32
+
+
+def test_repeat_execution(gpu_available):
+    # two executions of one plan give identical results (no state leaks)
+    tilts, proj, _ = small_series(px=64, py=4, vol_z=20)
+    print("case repeat", flush=True)
+    plan = Plan(64, 4, tilts, 20, L.METHOD_SART, sart_iters=2, sart_subsets=2)
+    a = plan.execute_host(proj)
+    b = plan.execute_host(proj)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_larger_series_sampled_slices(gpu_available):
+    # raw 64 x 512 with tilt axis 85 deg + shifts (aligned 512 wide, 64 rows),
+    # 41 tilts, VolZ 128, benchmark SART options; sampled slices vs the oracle
+    tilts = phantom.tilt_range(-60, 60, 3)
+    raw, shifts = phantom.tilt_series(31, 64, 512, tilts, 128, n_ellipsoids=80, axis_min=4,
+                                      axis_max=30, tilt_axis=85.0, max_shift=3.0,
+                                      noise_frac=0.1)
+    axes = np.full(len(tilts), 85.0, np.float32)
+    print("case larger series", flush=True)
+    plan = Plan(64, 512, tilts, 128, L.METHOD_SART, tilt_axis=axes, shifts=shifts,
+                sart_iters=5, sart_subsets=5, relax=0.3, subset_order=L.SUBSETS_INTERLEAVED,
+                positivity=False, fill_seed=31)
+    gpu = plan.execute_host(raw)
+    aligned = oracle.prestage(raw, axes, shifts, 1, 31)
+    assert aligned.shape[1:] == (64, 512) and gpu.shape == (64, 128, 512)
+    sl = [0, 13, 31, 40, 63]
+    ref = oracle.recon_slices(aligned, tilts, 1, 128, sl, iters=5, n_subsets=5, relax=0.3,
+                              order=1, positivity=0, seed=31)
+    err = rel_l2(gpu[sl], ref)
+    print("rel_l2", err, "shape", gpu.shape)
+    assert err <= TOL32
