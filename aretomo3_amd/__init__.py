@@ -10,8 +10,8 @@ reference operators (``recon``) and the synthetic-data generator
 """
 from . import _lib
 from ._lib import AtrError
-from .recon import (Batch, DeviceArray, DoSartRecon, DoWbpRecon, Plan, TomoSart, TomoWbp,
-                    device_count, synchronize)
+from .recon import (Batch, DeviceArray, DoSartRecon, DoWbpRecon, PinnedArray, Plan, Server,
+                    TomoSart, TomoWbp, device_count, series_desc, synchronize)
 
-__all__ = ["AtrError", "Batch", "DeviceArray", "DoSartRecon", "DoWbpRecon", "Plan", "TomoSart",
-           "TomoWbp", "device_count", "synchronize", "_lib"]
+__all__ = ["AtrError", "Batch", "DeviceArray", "DoSartRecon", "DoWbpRecon", "PinnedArray", "Plan",
+           "Server", "TomoSart", "TomoWbp", "device_count", "series_desc", "synchronize", "_lib"]

@@ -34,6 +34,8 @@ EXPORTS = [
     "atr_event_destroy", "atr_event_record", "atr_event_elapsed_ms",
     "atr_batch_create", "atr_batch_execute", "atr_batch_set_profiling", "atr_batch_kernel_stats",
     "atr_batch_reset_stats", "atr_batch_destroy",
+    "atr_server_config_init", "atr_server_create", "atr_server_submit", "atr_server_wait",
+    "atr_server_stats", "atr_server_destroy", "atr_host_alloc", "atr_host_free",
 ]
 BATCH_GRAPH = 1
 
@@ -64,6 +66,19 @@ class SeriesDesc(ctypes.Structure):
         ("out_layout", ctypes.c_int),
         ("fill_seed", ctypes.c_uint64),
     ]
+
+
+class ServerConfig(ctypes.Structure):
+    """Mirror of atr_server_config."""
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_active", ctypes.c_int),
+                ("storage", ctypes.c_int)]
+
+
+class RequestInfo(ctypes.Structure):
+    """Mirror of atr_request_info."""
+    _fields_ = [("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("status", ctypes.c_int)]
 
 
 class AtrError(RuntimeError):
@@ -125,6 +140,15 @@ def lib():
         "atr_batch_kernel_stats": (i, [vp, i, pi, ctypes.POINTER(ctypes.c_double)]),
         "atr_batch_reset_stats": (i, [vp]),
         "atr_batch_destroy": (None, [vp]),
+        "atr_server_config_init": (None, [ctypes.POINTER(ServerConfig)]),
+        "atr_server_create": (i, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
+        "atr_server_submit": (i, [vp, ctypes.POINTER(SeriesDesc), i, vp, vp, vp,
+                                  ctypes.POINTER(u64)]),
+        "atr_server_wait": (i, [vp, u64, ctypes.POINTER(RequestInfo)]),
+        "atr_server_stats": (i, [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+        "atr_server_destroy": (None, [vp]),
+        "atr_host_alloc": (i, [ctypes.POINTER(vp), sz]),
+        "atr_host_free": (i, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -78,40 +78,52 @@ def synchronize():
     check(lib().atr_synchronize())
 
 
+def series_desc(proj_x, proj_y, tilts, vol_z, method=L.METHOD_SART, *, tilt_axis=None,
+                shifts=None, random_fill=True, exclude=None, sart_iters=20, sart_subsets=1,
+                sart_start_tilt=0, sart_n_tilts=0, relax=-1.0,
+                subset_order=L.SUBSETS_CONTIGUOUS, positivity=True, storage=L.STORAGE_F32,
+                layout=L.LAYOUT_XZY, fill_seed=0, tilt_stride=0, row_stride=0):
+    """An atr_series_desc and the arrays it points to (keep both alive)."""
+    t = np.ascontiguousarray(tilts, np.float32)
+    n = len(t)
+    keep = [t]
+    d = L.SeriesDesc()
+    lib().atr_series_desc_init(ctypes.byref(d))
+    d.proj_x, d.proj_y, d.n_tilts = int(proj_x), int(proj_y), n
+    d.tilt_stride, d.row_stride = int(tilt_stride), int(row_stride)
+    d.tilt_deg = t.ctypes.data
+    if tilt_axis is not None:
+        ax = np.ascontiguousarray(np.broadcast_to(np.asarray(tilt_axis, np.float32), (n,)))
+        keep.append(ax)
+        d.tilt_axis_deg = ax.ctypes.data
+    if shifts is not None:
+        sh = np.ascontiguousarray(shifts, np.float32).reshape(-1)
+        keep.append(sh)
+        d.shift_xy = sh.ctypes.data
+    d.random_fill = 1 if random_fill else 0
+    if exclude is not None:
+        ex = np.ascontiguousarray(exclude, np.uint8)
+        keep.append(ex)
+        d.exclude = ex.ctypes.data
+    d.vol_z = int(vol_z)
+    d.method = int(method)
+    d.sart_iters, d.sart_subsets = int(sart_iters), int(sart_subsets)
+    d.sart_start_tilt, d.sart_n_tilts = int(sart_start_tilt), int(sart_n_tilts)
+    d.sart_relax = float(relax)
+    d.subset_order, d.positivity = int(subset_order), 1 if positivity else 0
+    d.storage, d.out_layout = int(storage), int(layout)
+    d.fill_seed = int(fill_seed)
+    return d, keep
+
+
 class Plan:
     """One reconstruction plan (atr_plan_create).  Arrays given here are
     copied into the descriptor; the plan keeps them alive."""
 
-    def __init__(self, proj_x, proj_y, tilts, vol_z, method=L.METHOD_SART, *, tilt_axis=None,
-                 shifts=None, random_fill=True, exclude=None, sart_iters=20, sart_subsets=1,
-                 sart_start_tilt=0, sart_n_tilts=0, relax=-1.0,
-                 subset_order=L.SUBSETS_CONTIGUOUS, positivity=True, storage=L.STORAGE_F32,
-                 layout=L.LAYOUT_XZY, fill_seed=0, tilt_stride=0, row_stride=0):
-        self._tilts = np.ascontiguousarray(tilts, np.float32)
-        n = len(self._tilts)
-        self._axis = None
-        if tilt_axis is not None:
-            ax = np.asarray(tilt_axis, np.float32)
-            self._axis = np.ascontiguousarray(np.broadcast_to(ax, (n,)), np.float32)
-        self._shifts = None if shifts is None else np.ascontiguousarray(shifts, np.float32).reshape(-1)
-        self._excl = None if exclude is None else np.ascontiguousarray(exclude, np.uint8)
-        d = L.SeriesDesc()
-        lib().atr_series_desc_init(ctypes.byref(d))
-        d.proj_x, d.proj_y, d.n_tilts = int(proj_x), int(proj_y), n
-        d.tilt_stride, d.row_stride = int(tilt_stride), int(row_stride)
-        d.tilt_deg = self._tilts.ctypes.data
-        d.tilt_axis_deg = None if self._axis is None else self._axis.ctypes.data
-        d.shift_xy = None if self._shifts is None else self._shifts.ctypes.data
-        d.random_fill = 1 if random_fill else 0
-        d.exclude = None if self._excl is None else self._excl.ctypes.data
-        d.vol_z = int(vol_z)
-        d.method = int(method)
-        d.sart_iters, d.sart_subsets = int(sart_iters), int(sart_subsets)
-        d.sart_start_tilt, d.sart_n_tilts = int(sart_start_tilt), int(sart_n_tilts)
-        d.sart_relax = float(relax)
-        d.subset_order, d.positivity = int(subset_order), 1 if positivity else 0
-        d.storage, d.out_layout = int(storage), int(layout)
-        d.fill_seed = int(fill_seed)
+    def __init__(self, proj_x, proj_y, tilts, vol_z, method=L.METHOD_SART, **kw):
+        d, self._keep = series_desc(proj_x, proj_y, tilts, vol_z, method, **kw)
+        n = d.n_tilts
+        layout = d.out_layout
         self.desc = d
         self.layout = int(layout)
         h = ctypes.c_void_p()
@@ -126,24 +138,11 @@ class Plan:
             (d.row_stride or proj_x) * (proj_y - 1) + proj_x
 
     @staticmethod
-    def estimate_bytes(**kw):
+    def estimate_bytes(proj_x, proj_y, tilts, vol_z, method=L.METHOD_SART, **kw):
         """Device bytes a plan with these arguments would allocate."""
-        p = Plan.__new__(Plan)
-        tilts = np.ascontiguousarray(kw["tilts"], np.float32)
-        d = L.SeriesDesc()
-        lib().atr_series_desc_init(ctypes.byref(d))
-        d.proj_x, d.proj_y, d.n_tilts = kw["proj_x"], kw["proj_y"], len(tilts)
-        d.tilt_deg = tilts.ctypes.data
-        d.vol_z, d.method = kw["vol_z"], kw.get("method", L.METHOD_SART)
-        d.sart_iters, d.sart_subsets = kw.get("sart_iters", 20), kw.get("sart_subsets", 1)
-        d.storage = kw.get("storage", L.STORAGE_F32)
-        axis = None
-        if kw.get("tilt_axis") is not None:
-            axis = np.full(len(tilts), kw["tilt_axis"], np.float32)
-            d.tilt_axis_deg = axis.ctypes.data
+        d, keep = series_desc(proj_x, proj_y, tilts, vol_z, method, **kw)
         b = ctypes.c_size_t()
         check(lib().atr_estimate_device_bytes(ctypes.byref(d), ctypes.byref(b)))
-        del p
         return b.value
 
     @property
@@ -229,6 +228,100 @@ class Batch:
     def close(self):
         if getattr(self, "handle", None):
             lib().atr_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """Page-locked host memory (atr_host_alloc) viewed as a numpy array, so
+    host<->device copies of the serving loop overlap the GPU work."""
+
+    def __init__(self, shape, dtype=np.float32):
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = ctypes.c_void_p()
+        check(lib().atr_host_alloc(ctypes.byref(p), max(nbytes, 4)))
+        self.ptr = p.value
+        buf = (ctypes.c_char * max(nbytes, 4)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            check(lib().atr_host_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (atr_server_*): submit SART series (optionally with
+    WBP) as they arrive; each is admitted by the HBM budget and reconstructed
+    together with the others on the GPU.  Output arrays are allocated here."""
+
+    def __init__(self, hbm_budget=0, max_active=None, storage=L.STORAGE_F16):
+        cfg = L.ServerConfig()
+        lib().atr_server_config_init(ctypes.byref(cfg))
+        cfg.hbm_budget_bytes, cfg.storage = int(hbm_budget), int(storage)
+        if max_active is not None:
+            cfg.max_active = int(max_active)
+        h = ctypes.c_void_p()
+        check(lib().atr_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h.value
+        self._live = {}
+
+    def submit(self, proj, tilts, vol_z, with_wbp=False, out=None, wbp_out=None, **kw):
+        """proj: float32 [n][y][x] (numpy or PinnedArray.array, kept alive until
+        wait); returns the request id.  kw: the Plan options of a SART series."""
+        proj = np.ascontiguousarray(proj, np.float32)
+        n, py, px = proj.shape
+        d, keep = series_desc(px, py, tilts, vol_z, L.METHOD_SART, **kw)
+        ax, ay = px, py
+        if kw.get("tilt_axis") is not None:
+            ax0 = float(np.asarray(kw["tilt_axis"], np.float32).reshape(-1)[0])
+            if abs(np.sin(ax0 * 3.14 / 180.0)) > 0.707:
+                ax, ay = py, px
+        layout = kw.get("layout", L.LAYOUT_XZY)
+        vx = ax // 2 * 2
+        shape = (vol_z, ay, vx) if layout in (L.LAYOUT_XYZ, L.LAYOUT_XYZ_NOFLIP) else (ay, vol_z, vx)
+        vol = out if out is not None else np.empty(shape, np.float32)
+        wbp = None
+        if with_wbp:
+            wbp = wbp_out if wbp_out is not None else np.empty(shape, np.float32)
+        rid = ctypes.c_uint64()
+        check(lib().atr_server_submit(self.handle, ctypes.byref(d), 1 if with_wbp else 0,
+                                      proj.ctypes.data, vol.ctypes.data,
+                                      wbp.ctypes.data if wbp is not None else None,
+                                      ctypes.byref(rid)))
+        self._live[rid.value] = (proj, vol, wbp, keep)
+        return rid.value
+
+    def wait(self, rid):
+        """Blocks until request rid is done: (sart volume, wbp volume or None, info)."""
+        info = L.RequestInfo()
+        try:
+            check(lib().atr_server_wait(self.handle, rid, ctypes.byref(info)))
+        finally:
+            proj, vol, wbp, keep = self._live.pop(rid)
+        return vol, wbp, {"submit_ms": info.submit_ms, "admit_ms": info.admit_ms,
+                          "done_ms": info.done_ms, "latency_ms": info.latency_ms}
+
+    def stats(self):
+        r, m = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().atr_server_stats(self.handle, ctypes.byref(r), ctypes.byref(m)))
+        return {"rounds": r.value, "max_batch": m.value}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().atr_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

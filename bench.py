@@ -19,11 +19,14 @@ Other configurations (--config): wbp512 (WBP, 512^2 x 41 tilts step 3, volume
 512x512x128, tilt axis 0, noise 5%), sart2048-f16 / sart2048-bf16 and
 wbp2048-f16 / wbp2048-bf16 (2048^2 x 61 tilts, volume 2048x2048x512, 2000
 ellipsoids of semi-axes 4-80, noise 10%, 10 SART iterations, projections
-stored in FP16 / BF16 with FP32 accumulation), small (a reduced SART case for
-quick checks; not a benchmark).
+stored in FP16 / BF16 with FP32 accumulation), serve (the serving loop: a
+Poisson stream of mixed-size WBP + SART requests, reporting p50/p99 latency and
+tomograms/s beside the plain per-series FP32 path), small (a reduced SART case
+for quick checks; not a benchmark).
 
 Prints one JSON line.  Options: --steps K --warmup W --gpus N (only 1),
---config NAME, --no-cpu-baseline, --cpu-slices S, --copy-probe MIB.
+--config NAME, --no-cpu-baseline, --cpu-slices S, --copy-probe MIB,
+--requests R and --rate L (serve).
 """
 import argparse
 import json
@@ -108,12 +111,154 @@ def describe(name, w, vol):
              alg, w["storage"].upper()))
 
 
+SERVE = dict(requests=24, rate=1.0, pool=8, seed=20261206, large_frac=0.1, iters=10, subsets=5)
+
+
+def serve_bench(args):
+    """The serving stream: open-loop Poisson arrivals of SART(10 iterations) + WBP
+    requests of mixed sizes (tilts {31,41,51,61} over +-60 deg, projections
+    {512,768,1024}^2, thickness {128,256,384}, tilt axis uniform 80-95 deg, plus
+    ~10% large requests 2048^2 x 61 x 512), FP16 projection storage, admission by
+    the device-memory budget.  Reports tomograms/s, p50/p99 latency and
+    Gvoxel-tilt updates/s, beside the plain per-series FP32 path on the same
+    requests (series one after another, no batching)."""
+    from aretomo3_amd import DeviceArray, PinnedArray, Plan, Server, synchronize
+    from aretomo3_amd import _lib as L
+    from aretomo3_amd import phantom
+    c = dict(SERVE)
+    c["requests"] = args.requests or c["requests"]
+    c["rate"] = args.rate or c["rate"]
+    rng = np.random.default_rng(c["seed"])
+    # a pool of distinct series; requests draw from it (the GPU work of a
+    # request depends on its sizes, not on the phantom)
+    pool = []
+    for k in range(c["pool"]):
+        large = k == 0
+        n = 61 if large else int(rng.choice([31, 41, 51, 61]))
+        px = 2048 if large else int(rng.choice([512, 768, 1024]))
+        vz = 512 if large else int(rng.choice([128, 256, 384]))
+        axis = float(rng.uniform(80.0, 95.0))
+        tilts = np.linspace(-60.0, 60.0, n).astype(np.float32)
+        t0 = time.time()
+        raw, shifts = phantom.tilt_series(c["seed"] + k, px, px, tilts, vz,
+                                          n_ellipsoids=int(rng.integers(200, 801)),
+                                          axis_min=4.0, axis_max=40.0, tilt_axis=axis, max_shift=3.0,
+                                          noise_frac=0.10, n_threads=16)
+        pin = PinnedArray(raw.shape)
+        pin.array[...] = raw
+        kw = dict(tilt_axis=np.full(n, axis, np.float32), shifts=shifts, sart_iters=c["iters"],
+                  sart_subsets=c["subsets"], relax=0.3, subset_order=L.SUBSETS_INTERLEAVED,
+                  positivity=False, fill_seed=c["seed"] + k)
+        ax = px
+        vx = ax // 2 * 2
+        updates = float(vx) * px * vz * n * (1 + 2 * c["iters"]) + float(vx) * px * vz * n
+        pool.append(dict(pin=pin, tilts=tilts, vz=vz, kw=kw, large=large, updates=updates,
+                         shape=raw.shape))
+        print(f"[bench] serve pool {k}: {raw.shape} vz {vz} in {time.time() - t0:.1f}s",
+              file=sys.stderr, flush=True)
+    # request stream: ~large_frac of requests are the large series
+    picks = [0 if rng.random() < c["large_frac"] else int(rng.integers(1, c["pool"]))
+             for _ in range(c["requests"])]
+    gaps = rng.exponential(1.0 / c["rate"], c["requests"])
+
+    # plain per-series FP32 path: each pool series alone (WBP + SART), timed
+    plain_s = []
+    for k, e in enumerate(pool):
+        n, py, px = e["shape"]
+        d_in = DeviceArray.from_numpy(e["pin"].array)
+        ps = Plan(px, py, e["tilts"], e["vz"], L.METHOD_SART, storage=L.STORAGE_F32, **e["kw"])
+        pw = Plan(px, py, e["tilts"], e["vz"], L.METHOD_WBP, storage=L.STORAGE_F32,
+                  tilt_axis=e["kw"]["tilt_axis"], shifts=e["kw"]["shifts"], fill_seed=e["kw"]["fill_seed"])
+        d_out = DeviceArray(4 * ps.vol_x * ps.vol_y * ps.vol_z)
+        d_w = DeviceArray(4 * pw.vol_x * pw.vol_y * pw.vol_z)
+        ps.execute(d_in, d_out)
+        pw.execute(d_in, d_w)
+        synchronize()
+        t0 = time.perf_counter()
+        h_out = np.empty(ps.volume_shape, np.float32)
+        d_in.upload(e["pin"].array)
+        pw.execute(d_in, d_w)
+        ps.execute(d_in, d_out)
+        synchronize()
+        h_out = d_out.to_numpy(ps.volume_shape)
+        h_w = d_w.to_numpy(pw.volume_shape)
+        plain_s.append(time.perf_counter() - t0)
+        ps.close(), pw.close()
+        del d_in, d_out, d_w, h_out, h_w
+    plain_total = sum(plain_s[k] for k in picks)
+
+    srv = Server(storage=L.STORAGE_F16)
+    # warm-up: one small request through the server
+    e = pool[1]
+    srv.wait(srv.submit(e["pin"].array, e["tilts"], e["vz"], with_wbp=True, **e["kw"]))
+    ids = []
+    t_start = time.perf_counter()
+    t_next = t_start
+    for k, g in zip(picks, gaps):
+        t_next += g
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        e = pool[k]
+        ids.append(srv.submit(e["pin"].array, e["tilts"], e["vz"], with_wbp=True, **e["kw"]))
+    lat = []
+    for rid in ids:
+        _, _, info = srv.wait(rid)
+        lat.append(info["latency_ms"])
+    makespan = time.perf_counter() - t_start
+    stats = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    updates = sum(pool[k]["updates"] for k in picks)
+    tps = len(picks) / makespan
+    plain_tps = len(picks) / plain_total
+    result = {
+        "metric": METRIC,
+        "value": round(updates / makespan / 1e9, 2),
+        "unit": "Gvoxel-tilt updates/s",
+        "n_gpus": 1,
+        "steps": len(picks),
+        "warmup": 1,
+        "ms_per_step": round(1e3 * makespan / len(picks), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantoms, analytic projections)",
+        "config": {
+            "workload": ("serve: %d requests, Poisson arrivals at %g/s, WBP + SART %d iterations x %d "
+                         "interleaved subsets (relax 0.3, positivity off) per request, tilts "
+                         "{31,41,51,61} over +-60 deg, projections {512,768,1024}^2, thickness "
+                         "{128,256,384}, tilt axis 80-95 deg + shifts +-3 px, ~%d%% large "
+                         "(2048^2 x 61 x 512), FP16 projection storage, FP32 accumulation; "
+                         "requests drawn from %d distinct seeded series" %
+                         (len(picks), c["rate"], c["iters"], c["subsets"], int(100 * c["large_frac"]),
+                          c["pool"])),
+            "projection_storage": "f16",
+        },
+        "serving": {
+            "tomograms_per_s": round(tps, 3),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 1),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 1),
+            "makespan_s": round(makespan, 3),
+            "rounds": stats["rounds"],
+            "max_series_per_round": stats["max_batch"],
+            "plain_fp32_per_series_tomograms_per_s": round(plain_tps, 3),
+            "speedup_vs_plain": round(tps / plain_tps, 3),
+            "large_requests": int(sum(1 for k in picks if k == 0)),
+        },
+    }
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="sart1024", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="sart1024", choices=sorted(CONFIGS) + ["serve"])
+    ap.add_argument("--requests", type=int, default=0, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second")
     ap.add_argument("--small", action="store_true", help="same as --config small")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=0,
@@ -129,6 +274,9 @@ def main():
               "(the reconstruction would shard over y-slabs with no exchange between GPUs)",
               file=sys.stderr)
         sys.exit(2)
+    if args.config == "serve":
+        serve_bench(args)
+        return
     name = "small" if args.small else args.config
     w = CONFIGS[name]
 

@@ -188,6 +188,45 @@ atr_status atr_batch_kernel_stats(const atr_batch* batch, int kernel, int* launc
 atr_status atr_batch_reset_stats(atr_batch* batch);
 void atr_batch_destroy(atr_batch* batch);
 
+/* ---- serving: a device-resident request loop ------------------------------
+ * No single reference counterpart: the reference processes the series of a
+ * session one after another per GPU (AreTomo/CAreTomoMain.cpp:809-848, queue
+ * in CProcessThread.cpp).  A server admits SART requests (optionally with a WBP
+ * reconstruction of the same series) in arrival order while their device
+ * bytes fit the HBM budget, keeps each series' data on the GPU from
+ * pre-stage to output, and advances all admitted series together: every
+ * round is one persistent forward-projection and one persistent
+ * back-projection launch carrying one SART step of each series.  Requests
+ * admitted mid-run join at the next round.  Each result equals what the
+ * series gives alone. */
+typedef struct atr_server atr_server;
+typedef struct atr_server_config {
+    size_t hbm_budget_bytes; /* 0 = 90% of the free device memory at creation */
+    int max_active;          /* series reconstructed together at most (default 4), 0 = no limit */
+    int storage;             /* projection storage of every request (default F16) */
+} atr_server_config;
+typedef struct atr_request_info {
+    double submit_ms, admit_ms, done_ms; /* host clock, ms since the server started */
+    double latency_ms;                   /* done - submit */
+    int status;
+} atr_request_info;
+void atr_server_config_init(atr_server_config* cfg);
+atr_status atr_server_create(const atr_server_config* cfg, atr_server** server);
+/* desc: a SART series (its storage is replaced by the server's); the
+ * descriptor and its arrays are copied.  proj_host, vol_host (and wbp_host
+ * when with_wbp) must stay valid until atr_server_wait returns for the id;
+ * pinned memory from atr_host_alloc lets the copies overlap the work. */
+atr_status atr_server_submit(atr_server* server, const atr_series_desc* desc, int with_wbp,
+                             const float* proj_host, float* vol_host, float* wbp_host,
+                             uint64_t* id);
+/* Blocks until the request is done; info may be NULL. */
+atr_status atr_server_wait(atr_server* server, uint64_t id, atr_request_info* info);
+atr_status atr_server_stats(atr_server* server, uint64_t* rounds, uint64_t* max_batch);
+/* Finishes every submitted request, then frees the server. */
+void atr_server_destroy(atr_server* server);
+atr_status atr_host_alloc(void** ptr, size_t bytes);
+atr_status atr_host_free(void* ptr);
+
 /* ---- device helpers (plumbing for hosts without a HIP binding) --------- */
 atr_status atr_device_count(int* count);
 atr_status atr_set_device(int device);
