@@ -36,6 +36,7 @@ EXPORTS = [
     "atr_batch_reset_stats", "atr_batch_destroy",
     "atr_server_config_init", "atr_server_create", "atr_server_submit", "atr_server_wait",
     "atr_server_stats", "atr_server_destroy", "atr_host_alloc", "atr_host_free",
+    "atr_plan_autotune", "atr_plan_set_tile_shape",
 ]
 BATCH_GRAPH = 1
 
@@ -149,6 +150,8 @@ def lib():
         "atr_server_destroy": (None, [vp]),
         "atr_host_alloc": (i, [ctypes.POINTER(vp), sz]),
         "atr_host_free": (i, [vp]),
+        "atr_plan_autotune": (i, [vp, vp, pi, pi, pi]),
+        "atr_plan_set_tile_shape": (i, [vp, i, i, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

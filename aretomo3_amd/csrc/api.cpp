@@ -12,7 +12,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "atr_internal.h"
 
@@ -188,6 +190,63 @@ static size_t plan_bytes_estimate(const Plan& p)
     return b;
 }
 
+template <class T>
+static atr_status reupload(T** dst, const std::vector<T>& src, size_t& bytes, size_t& old_bytes)
+{
+    if (*dst) {
+        ATR_HIP(hipFree(*dst));
+        bytes -= old_bytes;
+    }
+    old_bytes = sizeof(T) * src.size();
+    return upload(dst, src, bytes);
+}
+
+// The SART tables that depend on the tile shapes: back-projection tile
+// windows/classes for z-tiles of tz rows, forward-projection band tables and
+// wave schedules for bands of bz cell rows, and the per-step kernel
+// arguments.  Rebuilt by the autotuner for each candidate.
+static atr_status sart_tables(Plan& p, int tz, int chunk, int bz)
+{
+    Geometry& g = p.geo;
+    size_t& B = p.device_bytes;
+    atr_status st;
+    p.bp_tz = tz;
+    p.bp_chunk = chunk;
+    build_bp_tiles(g.cos_sin, g.bp_lohi, p.n, p.vol_x, p.vol_z, p.proj, tz, g.bp_tiles);
+    if (!build_fp_tables(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, bz, g))
+        return fail(ATR_ERR_INVALID_ARG, "SART needs every tilt in (-90, 90) degrees");
+    if ((st = reupload(&p.d_bp_tiles, g.bp_tiles, B, p.tab_bytes[0])) != ATR_OK) return st;
+    if ((st = reupload(&p.d_fp_range, g.fp_range, B, p.tab_bytes[1])) != ATR_OK) return st;
+    if ((st = reupload(&p.d_fp_geo_off, g.fp_geo_off, B, p.tab_bytes[2])) != ATR_OK) return st;
+    if ((st = reupload(&p.d_fp_len, g.fp_len, B, p.tab_bytes[3])) != ATR_OK) return st;
+    if ((st = reupload(&p.d_fp_geo, g.fp_geo, B, p.tab_bytes[4])) != ATR_OK) return st;
+    if ((st = reupload(&p.d_fp_desc, g.fp_desc, B, p.tab_bytes[5])) != ATR_OK) return st;
+    std::vector<int> sched;
+    p.subset_pair_off.clear();
+    for (int sb = 0; sb < p.n_subsets; sb++) {
+        p.subset_pair_off.push_back((int)sched.size());
+        const int f0 = p.subset_fp_off[sb], f1 = p.subset_fp_off[sb + 1];
+        fp_pair_schedule(g, p.subset_fp.data() + f0, f1 - f0, p.proj, fp_group(p.proj), sched);
+    }
+    if ((st = reupload(&p.d_fp_pairs, sched, B, p.tab_bytes[6])) != ATR_OK) return st;
+    // kernel arguments of every SART step, read by the kernels from device
+    // memory: [0] initial back-projection, [1 + s] subset s
+    const int n_all = (int)p.all_tilts.size();
+    std::vector<BpArgs> bpa;
+    bpa.push_back(bp_args(p, p.d_sino, p.n, p.d_lists, n_all, 1.0f, true));
+    std::vector<FpArgs> fpa;
+    for (int s = 0; s < p.n_subsets; s++) {
+        const int b0 = p.subset_off[s], b1 = p.subset_off[s + 1];
+        const int f0 = p.subset_fp_off[s], f1 = p.subset_fp_off[s + 1];
+        bpa.push_back(bp_args(p, p.d_diff, p.max_subset, p.d_lists + n_all + b0, b1 - b0, p.relax,
+                              false));
+        fpa.push_back(fp_args(p, p.d_fp_lists + f0, f1 - f0, p.d_fp_pairs + p.subset_pair_off[s]));
+    }
+    if ((st = reupload(&p.d_bp_args, bpa, B, p.tab_bytes[7])) != ATR_OK) return st;
+    if ((st = reupload(&p.d_fp_args, fpa, B, p.tab_bytes[8])) != ATR_OK) return st;
+    return ATR_OK;
+}
+
 static atr_status plan_build(Plan& p)
 {
     const atr_series_desc& d = p.desc;
@@ -236,27 +295,7 @@ static atr_status plan_build(Plan& p)
 
     if (p.method == ATR_METHOD_SART) {
         build_bp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.bp_lohi);
-        build_bp_tiles(g.cos_sin, g.bp_lohi, p.n, p.vol_x, p.vol_z, p.proj, g.bp_tiles);
-        if (!build_fp_tables(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, fp_band_rows(p.vol_x), g))
-            return fail(ATR_ERR_INVALID_ARG, "SART needs every tilt in (-90, 90) degrees");
         if ((st = upload(&p.d_bp_lohi, g.bp_lohi, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_bp_tiles, g.bp_tiles, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_range, g.fp_range, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_geo_off, g.fp_geo_off, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_len, g.fp_len, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_geo, g.fp_geo, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_desc, g.fp_desc, B)) != ATR_OK) return st;
-        {
-            std::vector<int> sched;
-            p.subset_pair_off.clear();
-            for (int sb = 0; sb < p.n_subsets; sb++) {
-                p.subset_pair_off.push_back((int)sched.size());
-                const int f0 = p.subset_fp_off[sb], f1 = p.subset_fp_off[sb + 1];
-                fp_pair_schedule(g, p.subset_fp.data() + f0, f1 - f0, p.proj, fp_group(p.proj),
-                                 sched);
-            }
-            if ((st = upload(&p.d_fp_pairs, sched, B)) != ATR_OK) return st;
-        }
         std::vector<TiltRow> lists(p.all_tilts);
         lists.insert(lists.end(), p.subset_tilts.begin(), p.subset_tilts.end());
         if ((st = upload(&p.d_lists, lists, B)) != ATR_OK) return st;
@@ -269,21 +308,9 @@ static atr_status plan_build(Plan& p)
         if ((st = dalloc(&p.d_diff, (size_t)p.vol_y * p.max_subset * p.sino_stride * sb, B)) != ATR_OK) return st;
         if ((st = dalloc((void**)&p.d_vflag, sizeof(int) * p.vol_y, B)) != ATR_OK) return st;
         if ((st = dalloc((void**)&p.d_vstats, sizeof(double) * 2 * p.vol_y, B)) != ATR_OK) return st;
-        // kernel arguments of every SART step, read by the kernels from device
-        // memory: [0] initial back-projection, [1 + s] subset s
-        const int n_all = (int)p.all_tilts.size();
-        std::vector<BpArgs> bpa;
-        bpa.push_back(bp_args(p, p.d_sino, p.n, p.d_lists, n_all, 1.0f, true));
-        std::vector<FpArgs> fpa;
-        for (int s = 0; s < p.n_subsets; s++) {
-            const int b0 = p.subset_off[s], b1 = p.subset_off[s + 1];
-            const int f0 = p.subset_fp_off[s], f1 = p.subset_fp_off[s + 1];
-            bpa.push_back(bp_args(p, p.d_diff, p.max_subset, p.d_lists + n_all + b0, b1 - b0,
-                                  p.relax, false));
-            fpa.push_back(fp_args(p, p.d_fp_lists + f0, f1 - f0, p.d_fp_pairs + p.subset_pair_off[s]));
-        }
-        if ((st = upload(&p.d_bp_args, bpa, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_args, fpa, B)) != ATR_OK) return st;
+        p.device_bytes = B;
+        if ((st = sart_tables(p, p.bp_tz, p.bp_chunk, fp_band_rows(p.vol_x))) != ATR_OK) return st;
+        B = p.device_bytes;
     } else {
         build_wbp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.wbp_lohi);
         if ((st = upload(&p.d_wbp_lohi, g.wbp_lohi, B)) != ATR_OK) return st;
@@ -441,6 +468,106 @@ static atr_status plan_execute(Plan& p, const float* in, float* out, int accumul
     return ATR_OK;
 }
 
+// ---- autotuner -------------------------------------------------------------
+// Times the candidate tile shapes on the plan's own geometry and keeps the
+// fastest: back-projection z rows per thread x tilts per LDS stage, then
+// forward-projection band rows.  Choices are cached per (sizes, tilts,
+// subsets, storage), so later plans of the same shape skip the timing.
+struct TuneKey {
+    int vol_x, vol_z, proj, n, n_subsets, storage;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(vol_x, vol_z, proj, n, n_subsets, storage) <
+               std::tie(o.vol_x, o.vol_z, o.proj, o.n, o.n_subsets, o.storage);
+    }
+};
+struct TuneChoice {
+    int tz, chunk, bz;
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneChoice> g_tune_cache;
+
+template <class F>
+static float time_ms(hipStream_t s, int reps, F&& fn)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    fn();  // warm-up
+    (void)hipEventRecord(a, s);
+    for (int r = 0; r < reps; r++) fn();
+    (void)hipEventRecord(b, s);
+    float ms = 0.0f;
+    (void)hipEventSynchronize(b);
+    (void)hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / reps;
+}
+
+static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_out, int* bz_out)
+{
+    if (p.method != ATR_METHOD_SART) {
+        if (tz_out) *tz_out = 0;
+        if (chunk_out) *chunk_out = 0;
+        if (bz_out) *bz_out = 0;
+        return ATR_OK;
+    }
+    const TuneKey key{p.vol_x, p.vol_z, p.proj, p.n, p.n_subsets, p.storage};
+    TuneChoice best{p.bp_tz, p.bp_chunk, p.geo.fp_bz};
+    bool cached = false;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            best = it->second;
+            cached = true;
+        }
+    }
+    atr_status st;
+    if (!cached) {
+        // defined contents for the timing runs (the next execute starts over)
+        const size_t sb = p.storage == ATR_STORAGE_F32 ? 4 : 2;
+        ATR_HIP(hipMemsetAsync(p.d_sino, 0, (size_t)p.vol_y * p.n * p.sino_stride * sb, s));
+        ATR_HIP(hipMemsetAsync(p.d_diff, 0, (size_t)p.vol_y * p.max_subset * p.sino_stride * sb, s));
+        ATR_HIP(hipMemsetAsync(p.d_vol, 0, sizeof(float) * p.vol_x * p.vol_y * p.vol_z, s));
+        ATR_HIP(hipMemsetAsync(p.d_sflag, 0, sizeof(int) * p.vol_y, s));
+        ATR_HIP(hipMemsetAsync(p.d_vflag, 0, sizeof(int) * p.vol_y, s));
+        const int bz0 = fp_band_rows(p.vol_x);
+        const bool has_bp = p.subset_off[1] > p.subset_off[0];
+        float best_bp = 1e30f;
+        const int shapes[4][2] = {{16, 16}, {16, 8}, {8, 16}, {8, 8}};
+        for (auto& sh : shapes) {
+            if ((st = sart_tables(p, sh[0], sh[1], bz0)) != ATR_OK) return st;
+            const float ms = time_ms(s, 3, [&] { launch_bp(p, has_bp ? 1 : 0, s); });
+            if (ms < best_bp) {
+                best_bp = ms;
+                best.tz = sh[0];
+                best.chunk = sh[1];
+            }
+        }
+        float best_fp = 1e30f;
+        std::vector<int> bzs = {bz0, std::max(2, bz0 * 2 / 3), std::max(2, bz0 / 2)};
+        for (int bz : bzs) {
+            if ((st = sart_tables(p, best.tz, best.chunk, bz)) != ATR_OK) return st;
+            const float ms = time_ms(s, 3, [&] { launch_fp_diff(p, 0, s); });
+            if (ms < best_fp) {
+                best_fp = ms;
+                best.bz = bz;
+            }
+        }
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = best;
+    }
+    if ((st = sart_tables(p, best.tz, best.chunk, best.bz)) != ATR_OK) return st;
+    if (tz_out) *tz_out = best.tz;
+    if (chunk_out) *chunk_out = best.chunk;
+    if (bz_out) *bz_out = best.bz;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "autotune");
+    return ATR_OK;
+}
+
 }  // namespace atr
 
 // ============================================================================
@@ -535,6 +662,25 @@ atr_status atr_execute_host(atr_plan* plan, const float* proj_host, float* vol_h
     if (d_in) (void)hipFree(d_in);
     if (d_out) (void)hipFree(d_out);
     return st;
+}
+
+atr_status atr_plan_set_tile_shape(atr_plan* plan, int bp_tz, int bp_chunk, int fp_rows)
+{
+    if (!plan) return fail(ATR_ERR_INVALID_ARG, "plan is NULL");
+    Plan& p = plan->p;
+    if (p.method != ATR_METHOD_SART) return fail(ATR_ERR_INVALID_ARG, "tile shapes are SART settings");
+    if ((bp_tz != 8 && bp_tz != 16) || (bp_chunk != 8 && bp_chunk != 16))
+        return fail(ATR_ERR_INVALID_ARG, "bp_tz and bp_chunk must be 8 or 16");
+    if (fp_rows < 2 || fp_rows > fp_band_rows(p.vol_x))
+        return fail(ATR_ERR_INVALID_ARG, "fp band rows outside [2, LDS limit]");
+    return sart_tables(p, bp_tz, bp_chunk, fp_rows);
+}
+
+atr_status atr_plan_autotune(atr_plan* plan, void* stream, int* bp_tz, int* bp_chunk,
+                             int* fp_band_rows_out)
+{
+    if (!plan) return fail(ATR_ERR_INVALID_ARG, "plan is NULL");
+    return plan_autotune(plan->p, (hipStream_t)stream, bp_tz, bp_chunk, fp_band_rows_out);
 }
 
 atr_status atr_plan_set_profiling(atr_plan* plan, int enable)

@@ -23,6 +23,7 @@ constexpr int BP_TX = 256;
 constexpr int BP_TZ = 16;
 constexpr int BP_CHUNK = 16;                 // tilts staged in LDS at once
 constexpr int BP_WIN = BP_TX + BP_TZ + 16;   // staged pairs per tilt
+constexpr int bp_win(int tz) { return BP_TX + tz + 16; }
 // Forward projection: a workgroup of FP_THREADS owns one slice and a group of
 // tilts; wave w takes the (tilt, 64-ray tile) pairs w, w+16, ... (FP_MAXP at
 // most), so every wave sees every tilt; the slice is streamed through LDS in
@@ -131,6 +132,8 @@ struct Plan {
     int* d_sflag = nullptr;           // [vol_y] sinogram slice holds sentinels
     int* d_vflag = nullptr;           // [vol_y] volume slice holds sentinels
     double* d_vstats = nullptr;       // [vol_y][2] fill statistics of volume slices
+    int bp_tz = BP_TZ, bp_chunk = BP_CHUNK;   // back-projection tile shape (autotuned)
+    size_t tab_bytes[9] = {0};        // sizes of the tile-shape dependent tables
     BpArgs* d_bp_args = nullptr;      // SART step arguments: initial BP, then per subset
     FpArgs* d_fp_args = nullptr;      // per subset
     float2* d_wbp_t = nullptr;        // [slab][2VolZ][VolX/2+1]
@@ -213,7 +216,7 @@ void cos_sin_table(const float* tilts, int n, std::vector<float>& cs);
 void build_bp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                         std::vector<int2>& lohi);
 void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi, int n, int vol_x,
-                    int vol_z, int proj, std::vector<int2>& tiles);
+                    int vol_z, int proj, int tz, std::vector<int2>& tiles);
 void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                          std::vector<int2>& lohi);
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,

@@ -224,6 +224,9 @@ atr_status atr_batch_create(atr_plan* const* plans, int n_plans, int flags, atr_
             return fail(ATR_ERR_INVALID_ARG, "atr_batch_create: batches hold SART plans only");
         if (plans[i]->p.storage != plans[0]->p.storage)
             return fail(ATR_ERR_INVALID_ARG, "atr_batch_create: plans differ in storage type");
+        if (plans[i]->p.bp_tz != BP_TZ || plans[i]->p.bp_chunk != BP_CHUNK)
+            return fail(ATR_ERR_INVALID_ARG,
+                        "atr_batch_create: batches use the default back-projection tile shape");
         for (int j = 0; j < i; j++)
             if (plans[j] == plans[i])
                 return fail(ATR_ERR_INVALID_ARG, "atr_batch_create: a plan appears twice");

@@ -127,15 +127,15 @@ void build_bp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_
 }
 
 // Per (tilt, z-tile, x-tile) of the SART back-projector: the start w0 of the
-// window of BP_WIN row samples the tile stages, and the tile class
+// window of BP_TX + tz + 16 row samples the tile stages, and the tile class
 // (1 = every voxel of the tile uses the tilt, 0 = none does, 2 = some do).
 // w0 = floor(min u over the tile corners) - 3 with u = fx cos + fz sin + P/2
 // in FP64; the kernel's FP32 coordinates relative to w0 then stay in
-// [2, BP_WIN - 2] (the tile spans at most (BP_TX-1)|cos| + (BP_TZ-1)|sin| in u).
+// [2, window - 2] (the tile spans at most (BP_TX-1)|cos| + (tz-1)|sin| in u).
 void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi, int n, int vol_x,
-                    int vol_z, int proj, std::vector<int2>& tiles)
+                    int vol_z, int proj, int tz, std::vector<int2>& tiles)
 {
-    int nzt = (vol_z + BP_TZ - 1) / BP_TZ, nxt = (vol_x + BP_TX - 1) / BP_TX;
+    int nzt = (vol_z + tz - 1) / tz, nxt = (vol_x + BP_TX - 1) / BP_TX;
     tiles.assign((size_t)n * nzt * nxt, int2{0, 0});
     for (int t = 0; t < n; t++) {
         const double c = cs[2 * t], s = cs[2 * t + 1];
@@ -143,13 +143,13 @@ void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi,
             for (int xt = 0; xt < nxt; xt++) {
                 int x0 = xt * BP_TX, x1 = std::min(x0 + BP_TX, vol_x) - 1;
                 bool full = true, any = false;
-                for (int z = zt * BP_TZ; z < std::min((zt + 1) * BP_TZ, vol_z); z++) {
+                for (int z = zt * tz; z < std::min((zt + 1) * tz, vol_z); z++) {
                     int2 r = lohi[(size_t)t * vol_z + z];
                     if (!(r.x <= x0 && r.y >= x1)) full = false;
                     if (!(r.x > x1 || r.y < x0)) any = true;
                 }
                 const double fx0 = x0 - vol_x * 0.5, fx1 = x1 - vol_x * 0.5;
-                const double fz0 = zt * BP_TZ - vol_z * 0.5, fz1 = fz0 + (BP_TZ - 1);
+                const double fz0 = zt * tz - vol_z * 0.5, fz1 = fz0 + (tz - 1);
                 const double umin = proj / 2.0 + std::min(c * fx0, c * fx1) + std::min(s * fz0, s * fz1);
                 tiles[((size_t)t * nzt + zt) * nxt + xt] =
                     int2{(int)std::floor(umin) - 3, full ? 1 : (any ? 2 : 0)};

@@ -162,8 +162,9 @@ __global__ __launch_bounds__(256) void k_weight(const float* __restrict__ in, si
 // ------------------------------------------------------------------------
 // SART back-projection (Recon/GBackProj.cu:12-60).
 //
-// Workgroup = BP_TX x BP_TZ voxels of one slice, lane = x, each thread walks
-// BP_TZ z-rows.  For each chunk of BP_CHUNK tilts the needed part of every
+// Workgroup = BP_TX x TZ voxels of one slice, lane = x, each thread walks
+// TZ z-rows (TZ = 16 by default, 8 a tuning choice).  For each chunk of CH
+// tilts (16 or 8) the needed part of every
 // sinogram row is staged in LDS as pairs (p[i], p[i+1]-p[i]) so that one
 // ds_read_b64 and one FMA give the linear interpolation.  Inclusion comes
 // from the FP64 interval tables: tiles the host classified as fully inside
@@ -172,10 +173,10 @@ __global__ __launch_bounds__(256) void k_weight(const float* __restrict__ in, si
 // order, as the reference.  Everything per tilt (list entry, cos/sin, the
 // tile's window start and class) is wave-uniform and read with scalar loads.
 // ------------------------------------------------------------------------
-// One tile: voxels x0 .. x0+BP_TX-1, z0 .. z0+BP_TZ-1 of slice y.  Ends
+// One tile: voxels x0 .. x0+BP_TX-1, z0 .. z0+TZ-1 of slice y.  Ends
 // after a barrier, with the LDS free for the next tile.
-template <typename S>
-__device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[BP_CHUNK][BP_WIN],
+template <typename S, int TZ, int CH>
+__device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_win(TZ)],
                                         int xt, int zt, int y)
 {
     const gptr<const S> sino = (gptr<const S>)A->sino;
@@ -193,21 +194,22 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[BP_CHUNK
     const int vol_x = A->vol_x, vol_z = A->vol_z;
     int* vflag = A->vflag;
     const int tid = threadIdx.x;
-    const int x0 = xt * BP_TX, z0 = zt * BP_TZ;
+    constexpr int WIN = bp_win(TZ);
+    const int x0 = xt * BP_TX, z0 = zt * TZ;
     const int x = x0 + tid;
     const int xc = min(x, vol_x - 1);
     const bool sent = sflag != nullptr && sflag[y] != 0;
     const gptr<float> vslice = vol + (size_t)y * vol_z * vol_x;
 
     // the voxels this thread updates, loaded now and consumed at the end
-    float vold[BP_TZ];
+    float vold[TZ];
 #pragma unroll
-    for (int k = 0; k < BP_TZ; k++) vold[k] = vslice[(size_t)min(z0 + k, vol_z - 1) * vol_x + xc];
+    for (int k = 0; k < TZ; k++) vold[k] = vslice[(size_t)min(z0 + k, vol_z - 1) * vol_x + xc];
 
-    float acc[BP_TZ];
-    int cnt[BP_TZ];
+    float acc[TZ];
+    int cnt[TZ];
 #pragma unroll
-    for (int k = 0; k < BP_TZ; k++) {
+    for (int k = 0; k < TZ; k++) {
         acc[k] = 0.0f;
         cnt[k] = 0;
     }
@@ -218,8 +220,8 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[BP_CHUNK
     const cptr<int2> tile_tab = tiles + (size_t)zt * n_xtiles + xt;
     const size_t tilt_step = (size_t)n_ztiles * n_xtiles;
 
-    for (int c0 = 0; c0 < n_list; c0 += BP_CHUNK) {
-        const int nc = min(BP_CHUNK, n_list - c0);
+    for (int c0 = 0; c0 < n_list; c0 += CH) {
+        const int nc = min(CH, n_list - c0);
         // ---- stage the row segments.  Pair i of tilt j is (q(g), q(g+1) - q(g))
         // at g = w0 + i, where q is the row extended linearly beyond its ends:
         // q(g) = p[ga] + (g - ga) (p[ga+1] - p[ga]), ga = clamp(g, 0, P-2).
@@ -240,7 +242,7 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[BP_CHUNK
                 for (int m = 0; m < 2; m++) {
                     const int g = w0 + tid + m * BP_TX;
                     const int ga = min(max(g, 0), proj - 2);
-                    if (m == 0 || tid < BP_WIN - BP_TX) {
+                    if (m == 0 || tid < WIN - BP_TX) {
                         pv[q][m] = ldg(row + (unsigned)ga);
                         rv[q][m] = ldg(row + (unsigned)ga + 1u);
                     }
@@ -252,7 +254,7 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[BP_CHUNK
                 if (h + q >= nc) break;
 #pragma unroll
                 for (int m = 0; m < 2; m++) {
-                    if (m == 1 && tid >= BP_WIN - BP_TX) break;
+                    if (m == 1 && tid >= WIN - BP_TX) break;
                     float p = pv[q][m], r = rv[q][m];
                     if constexpr (!std::is_same<S, float>::value) {
                         // narrow storage cannot hold -1e30: widen its empty marks
@@ -279,7 +281,7 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[BP_CHUNK
             if (tw.y == 1 && !sent) {
                 full_cnt += 1;
 #pragma unroll
-                for (int k = 0; k < BP_TZ; k++) {
+                for (int k = 0; k < TZ; k++) {
                     float u = fmaf((float)k, s, u0);
                     int i = (int)u;
                     float f = __builtin_amdgcn_fractf(u);
@@ -289,7 +291,7 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[BP_CHUNK
             } else {
                 const cptr<int2> lh = lohi + (size_t)tr.tilt * vol_z;
 #pragma unroll
-                for (int k = 0; k < BP_TZ; k++) {
+                for (int k = 0; k < TZ; k++) {
                     const int2 r = ldc(lh + min(z0 + k, vol_z - 1));
                     float u = fmaf((float)k, s, u0);
                     int i = (int)u;
@@ -307,7 +309,7 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[BP_CHUNK
     // ---- v = relax * sum / count + v, positivity, sentinel (GBackProj.cu:50-59) ----
     bool wrote_sent = false;
 #pragma unroll
-    for (int k = 0; k < BP_TZ; k++) {
+    for (int k = 0; k < TZ; k++) {
         const int z = z0 + k;
         const int count = full_cnt + cnt[k];
         float v = relax * acc[k] * __builtin_amdgcn_rcpf((float)count) + vold[k];
@@ -321,11 +323,11 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[BP_CHUNK
     if (wrote_sent) atomicOr(vflag + y, 1);
 }
 
-template <typename S>
+template <typename S, int TZ, int CH>
 __global__ __launch_bounds__(BP_TX) void k_back_proj(const BpArgs* __restrict__ a)
 {
-    __shared__ float2 pairs[BP_CHUNK][BP_WIN];
-    bp_tile<S>(as_cptr(a), pairs, blockIdx.x, blockIdx.y, blockIdx.z);
+    __shared__ float2 pairs[CH][bp_win(TZ)];
+    bp_tile<S, TZ, CH>(as_cptr(a), pairs, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Block-wide work queue: thread 0 takes the next item index, every thread
@@ -356,7 +358,7 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj_batch(const BpItem* __restr
         const int per_y = E[s].n_items_y;
         const int y = local / per_y, r = local - y * per_y;
         const int zt = r / E[s].a.n_xtiles, xt = r - zt * E[s].a.n_xtiles;
-        bp_tile<S>(&E[s].a, pairs, xt, zt, y);
+        bp_tile<S, BP_TZ, BP_CHUNK>(&E[s].a, pairs, xt, zt, y);
     }
 }
 
@@ -768,7 +770,7 @@ BpArgs bp_args(const Plan& p, const void* sino, int rows, const TiltRow* list, i
     a.cs = p.d_cs;
     a.lohi = p.d_bp_lohi;
     a.tiles = p.d_bp_tiles;
-    a.n_ztiles = (p.vol_z + BP_TZ - 1) / BP_TZ;
+    a.n_ztiles = (p.vol_z + p.bp_tz - 1) / p.bp_tz;
     a.n_xtiles = (p.vol_x + BP_TX - 1) / BP_TX;
     a.relax = relax;
     a.positivity = p.positivity;
@@ -780,14 +782,30 @@ BpArgs bp_args(const Plan& p, const void* sino, int rows, const TiltRow* list, i
     return a;
 }
 
+template <typename S, int TZ, int CH>
+static void launch_bp_v(const Plan& p, int step, hipStream_t s)
+{
+    const int nxt = (p.vol_x + BP_TX - 1) / BP_TX, nzt = (p.vol_z + TZ - 1) / TZ;
+    hipLaunchKernelGGL((k_back_proj<S, TZ, CH>), dim3(nxt, nzt, p.vol_y), dim3(BP_TX), 0, s,
+                       p.d_bp_args + step);
+}
+
+// Back-projection tile shapes the autotuner chooses from: z rows per thread
+// x tilts per LDS stage.
+template <typename S>
+static void launch_bp_t(const Plan& p, int step, hipStream_t s)
+{
+    if (p.bp_tz == 8 && p.bp_chunk == 8) launch_bp_v<S, 8, 8>(p, step, s);
+    else if (p.bp_tz == 8) launch_bp_v<S, 8, 16>(p, step, s);
+    else if (p.bp_chunk == 8) launch_bp_v<S, 16, 8>(p, step, s);
+    else launch_bp_v<S, 16, 16>(p, step, s);
+}
+
 void launch_bp(const Plan& p, int step, hipStream_t s)
 {
-    const int nxt = (p.vol_x + BP_TX - 1) / BP_TX, nzt = (p.vol_z + BP_TZ - 1) / BP_TZ;
-    dim3 grid(nxt, nzt, p.vol_y);
-    const BpArgs* a = p.d_bp_args + step;
-    if (p.storage == ATR_STORAGE_F16) hipLaunchKernelGGL(k_back_proj<_Float16>, grid, dim3(BP_TX), 0, s, a);
-    else if (p.storage == ATR_STORAGE_BF16) hipLaunchKernelGGL(k_back_proj<bf16_t>, grid, dim3(BP_TX), 0, s, a);
-    else hipLaunchKernelGGL(k_back_proj<float>, grid, dim3(BP_TX), 0, s, a);
+    if (p.storage == ATR_STORAGE_F16) launch_bp_t<_Float16>(p, step, s);
+    else if (p.storage == ATR_STORAGE_BF16) launch_bp_t<bf16_t>(p, step, s);
+    else launch_bp_t<float>(p, step, s);
 }
 
 void launch_bp_batch(int storage, const BpItem* items, int n_items, int total, int* counter,

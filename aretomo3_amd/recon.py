@@ -145,6 +145,17 @@ class Plan:
         check(lib().atr_estimate_device_bytes(ctypes.byref(d), ctypes.byref(b)))
         return b.value
 
+    def autotune(self, stream=None):
+        """Times the candidate tile shapes on this plan and keeps the fastest;
+        returns (bp z rows, bp tilts per stage, fp band rows)."""
+        tz, ch, bz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().atr_plan_autotune(self.handle, stream, ctypes.byref(tz), ctypes.byref(ch),
+                                      ctypes.byref(bz)))
+        return tz.value, ch.value, bz.value
+
+    def set_tile_shape(self, bp_tz, bp_chunk, fp_band_rows):
+        check(lib().atr_plan_set_tile_shape(self.handle, bp_tz, bp_chunk, fp_band_rows))
+
     @property
     def volume_shape(self):
         if self.layout in (L.LAYOUT_XYZ, L.LAYOUT_XYZ_NOFLIP):

@@ -141,6 +141,21 @@ atr_status atr_execute_host(atr_plan* plan, const float* proj_host, float* vol_h
  * enabled with atr_plan_set_profiling(plan, 1).  kernel: 0 = SART BP,
  * 1 = FP+diff, 2 = weighting/filter, 3 = WBP BP (2x grid + row FFT),
  * 4 = pre-stage, 5 = output/fill, 6 = WBP z-decimation, 7 = WBP row C2R. */
+/* Autotuner (no reference counterpart; the reference's tile shapes are fixed
+ * launch constants, Recon/GBackProj.cu:79-82, Recon/GForProj.cu:106-110):
+ * times the candidate tile shapes of a SART plan on its own geometry -- back-
+ * projection z rows per thread {16, 8} x tilts per LDS stage {16, 8}, then
+ * forward-projection band rows -- and keeps the fastest.  Choices are cached
+ * per shape within the process.  Every voxel and every ray sums its terms
+ * in the same order whatever the shape; only the FP32 rounding of the back-
+ * projection coordinate (taken relative to the tile's window) can differ, at
+ * the 1e-7 level.  Outputs may be NULL.  A tuned plan with a non-default back-projection
+ * shape cannot join a batch or be served. */
+atr_status atr_plan_autotune(atr_plan* plan, void* stream, int* bp_tz, int* bp_chunk,
+                             int* fp_band_rows);
+/* Sets the tile shapes by hand: bp_tz, bp_chunk in {8, 16}; fp_band_rows in
+ * [2, the largest band that fits in LDS]. */
+atr_status atr_plan_set_tile_shape(atr_plan* plan, int bp_tz, int bp_chunk, int fp_band_rows);
 atr_status atr_plan_set_profiling(atr_plan* plan, int enable);
 atr_status atr_plan_kernel_stats(const atr_plan* plan, int kernel, int* launches,
                                  double* total_ms);
