@@ -226,7 +226,8 @@ static atr_status sart_tables(Plan& p, int tz, int chunk, int bz)
     for (int sb = 0; sb < p.n_subsets; sb++) {
         p.subset_pair_off.push_back((int)sched.size());
         const int f0 = p.subset_fp_off[sb], f1 = p.subset_fp_off[sb + 1];
-        fp_pair_schedule(g, p.subset_fp.data() + f0, f1 - f0, p.proj, fp_group(p.proj), sched);
+        fp_pair_schedule(g, p.subset_fp.data() + f0, f1 - f0, p.proj, fp_group(p.proj, f1 - f0),
+                         sched);
     }
     if ((st = reupload(&p.d_fp_pairs, sched, B, p.tab_bytes[6])) != ATR_OK) return st;
     // kernel arguments of every SART step, read by the kernels from device

@@ -345,12 +345,15 @@ int fp_band_rows(int vol_x)
 
 // (tilt-in-group, tile) of every pair index p = wave + 16 i of the forward
 // projector: j << 16 | tile.
-// Tilts per forward-projection workgroup: all FP_WAVES * FP_MAXP pair slots
-// filled with 64-ray tiles.
-int fp_group(int proj)
+// Tilts per forward-projection workgroup for a list of n_list tilts: at most
+// the FP_WAVES * FP_MAXP pair slots of 64-ray tiles, and the list split into
+// groups of equal size (13 tilts with room for 8 -> 7 + 6, not 8 + 5).
+int fp_group(int proj, int n_list)
 {
     const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
-    return std::max(1, FP_WAVES * FP_MAXP / ntile);
+    const int cap = std::max(1, FP_WAVES * FP_MAXP / ntile);
+    const int groups = (std::max(n_list, 1) + cap - 1) / cap;
+    return (std::max(n_list, 1) + groups - 1) / groups;
 }
 
 // Assignment of the (tilt, 64-ray tile) pairs of one forward-projection

@@ -828,7 +828,7 @@ FpArgs fp_args(const Plan& p, const int* fp_list, int n_list, const int* sched)
     a.vflag = p.d_vflag;
     a.fp_list = fp_list;
     a.n_list = n_list;
-    a.group = fp_group(p.proj);
+    a.group = fp_group(p.proj, n_list);
     a.sched = sched;
     a.cs = p.d_cs;
     a.fp_range = p.d_fp_range;
@@ -868,7 +868,7 @@ static void launch_fp_t(const Plan& p, int subset, hipStream_t s)
     allow_big_lds(k_for_proj_diff<S, false>, done0);
     allow_big_lds(k_for_proj_diff<S, true>, done1);
     const int n_list = p.subset_fp_off[subset + 1] - p.subset_fp_off[subset];
-    const int group = fp_group(p.proj);
+    const int group = fp_group(p.proj, n_list);
     dim3 grid((n_list + group - 1) / group, p.vol_y);
     const size_t shm = fp_lds_bytes(p.geo.fp_bz, p.vol_x);
     const FpArgs* a = p.d_fp_args + subset;

@@ -261,6 +261,7 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second")
     ap.add_argument("--small", action="store_true", help="same as --config small")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--cpu-slices", type=int, default=0,
                     help="y-slices the CPU baseline reconstructs (0 = the config's default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -307,6 +308,12 @@ def main():
         plan = Plan(w["px"], w["px"], tilts, w["vol_z"], L.METHOD_WBP, tilt_axis=axes,
                     shifts=shifts, storage=storage, fill_seed=w["seed"])
     vol_x, vol_y, vol_z = plan.vol_x, plan.vol_y, plan.vol_z
+    tuned = None
+    if sart and not args.no_tune:
+        t0 = time.time()
+        tuned = plan.autotune()
+        print(f"[bench] autotuned tile shape (bp z rows, bp tilts/stage, fp band rows) = {tuned} "
+              f"in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     d_raw = DeviceArray.from_numpy(raw)
     d_vol = DeviceArray(4 * vol_x * vol_y * vol_z)
     print(f"[bench] plan {plan.device_bytes / 2**30:.2f} GiB, volume {vol_x}x{vol_y}x{vol_z}",
@@ -375,6 +382,8 @@ def main():
             "workload": describe(name, w, (vol_x, vol_y, vol_z)),
             "updates_per_step": updates,
             "projection_storage": w["storage"],
+            "tile_shape": None if tuned is None else {"bp_z_rows": tuned[0], "bp_tilts_per_stage": tuned[1],
+                                                      "fp_band_rows": tuned[2], "source": "autotuner"},
         },
         "roofline": {
             "kernel": rkernel,
