@@ -162,7 +162,8 @@ static atr_status plan_sizes(const atr_series_desc* d, Plan& p)
         if (p.proj > 4096 || p.vol_x > 2048 || p.vol_z > 2048)
             return fail(ATR_ERR_UNSUPPORTED_SIZE, "WBP supports ProjX <= 4096, VolZ <= 2048");
         // slab of slices whose spectra are buffered between the WBP kernels
-        size_t per_slice = (size_t)3 * p.vol_z * (p.vol_x / 2 + 1) * sizeof(float2);
+        size_t per_slice = (size_t)3 * p.vol_z * (p.vol_x / 2 + 1) * sizeof(float2) +
+                           (size_t)4 * p.vol_z * p.vol_x * sizeof(float);
         size_t budget = (size_t)1 << 30;
         p.wbp_slab = (int)std::max<size_t>(1, std::min<size_t>(p.vol_y, budget / per_slice));
     }
@@ -185,7 +186,9 @@ static size_t plan_bytes_estimate(const Plan& p)
         b += n * (size_t)(p.vol_z * 2.5 + 16) * 16;  // FP sample tables (bounded by 60 deg)
     } else {
         b += (size_t)p.wbp_slab * 3 * p.vol_z * (p.vol_x / 2 + 1) * sizeof(float2);
-        b += n * 2 * p.vol_z * 8;
+        b += (size_t)p.wbp_slab * 4 * p.vol_z * p.vol_x * sizeof(float);
+        b += n * 2 * p.vol_z * 8 + n * 16;
+        b += n * (size_t)((2 * p.vol_z + BP_TZ - 1) / BP_TZ) * ((2 * p.vol_x + BP_TX - 1) / BP_TX) * 8;
     }
     return b;
 }
@@ -212,7 +215,7 @@ static atr_status sart_tables(Plan& p, int tz, int chunk, int bz)
     atr_status st;
     p.bp_tz = tz;
     p.bp_chunk = chunk;
-    build_bp_tiles(g.cos_sin, g.bp_lohi, p.n, p.vol_x, p.vol_z, p.proj, tz, g.bp_tiles);
+    build_bp_tiles(g.cos_sin, g.bp_lohi, p.n, p.vol_x, p.vol_z, 1.0, p.proj / 2.0, tz, g.bp_tiles);
     if (!build_fp_tables(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, bz, g))
         return fail(ATR_ERR_INVALID_ARG, "SART needs every tilt in (-90, 90) degrees");
     if ((st = reupload(&p.d_bp_tiles, g.bp_tiles, B, p.tab_bytes[0])) != ATR_OK) return st;
@@ -315,6 +318,44 @@ static atr_status plan_build(Plan& p)
     } else {
         build_wbp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.wbp_lohi);
         if ((st = upload(&p.d_wbp_lohi, g.wbp_lohi, B)) != ATR_OK) return st;
+        {
+            // 2x-grid back-projection: tile windows/classes, the tilt list
+            // (excluded tilts left out, GBackProjWbp.cu:31), per-slab arguments
+            std::vector<int2> wt;
+            build_bp_tiles(g.cos_sin, g.wbp_lohi, p.n, 2 * p.vol_x, 2 * p.vol_z, 0.5,
+                           (double)(p.proj / 2) + 0.5, BP_TZ, wt);
+            if ((st = upload(&p.d_wbp_tiles, wt, B)) != ATR_OK) return st;
+            if ((st = upload(&p.d_lists, p.all_tilts, B)) != ATR_OK) return st;
+            if ((st = dalloc((void**)&p.d_wbp_grid,
+                             (size_t)p.wbp_slab * 4 * p.vol_z * p.vol_x * sizeof(float), B)) != ATR_OK)
+                return st;
+            std::vector<BpArgs> args;
+            for (int y0 = 0; y0 < p.vol_y; y0 += p.wbp_slab) {
+                BpArgs a{};
+                a.sino = (const char*)p.d_sino + (size_t)y0 * p.n * p.sino_stride * storage_bytes(p.storage);
+                a.rows_per_slice = p.n;
+                a.stride = p.sino_stride;
+                a.proj = p.proj;
+                a.list = p.d_lists;
+                a.n_list = (int)p.all_tilts.size();
+                a.cs = p.d_cs;
+                a.lohi = p.d_wbp_lohi;
+                a.tiles = p.d_wbp_tiles;
+                a.n_ztiles = (2 * p.vol_z + BP_TZ - 1) / BP_TZ;
+                a.n_xtiles = (2 * p.vol_x + BP_TX - 1) / BP_TX;
+                a.relax = 1.0f;
+                a.positivity = 0;
+                a.scale = 0.5f;
+                a.offset = (float)(p.proj / 2) + 0.5f;
+                a.sflag = p.d_sflag + y0;
+                a.vol = p.d_wbp_grid;
+                a.vol_x = 2 * p.vol_x;
+                a.vol_z = 2 * p.vol_z;
+                a.vflag = nullptr;
+                args.push_back(a);
+            }
+            if ((st = upload(&p.d_bp_args, args, B)) != ATR_OK) return st;
+        }
         if ((st = upload(&p.d_filter, ramp_filter(p.proj), B)) != ATR_OK) return st;
         make_fft(p.proj, p.fft_proj);
         make_fft(2 * p.vol_x, p.fft_x2);
@@ -345,7 +386,8 @@ static void plan_free(Plan& p)
                     p.d_fill_aln, p.d_fill_vol, p.d_filter, p.d_tw_proj, p.d_tw_x2, p.d_tw_z2,
                     p.d_tw_z, p.d_tw_x, p.d_axis_cs, p.d_shifts, p.d_excl, p.d_aligned,
                     p.d_row_stats, p.d_tilt_stats, p.d_sino, p.d_diff, p.d_vol, p.d_sflag,
-                    p.d_vflag, p.d_vstats, p.d_wbp_t, p.d_wbp_u, p.d_bp_args, p.d_fp_args};
+                    p.d_vflag, p.d_vstats, p.d_wbp_t, p.d_wbp_u, p.d_bp_args, p.d_fp_args,
+                    p.d_wbp_grid, p.d_wbp_tiles};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& e : p.ev_pool) (void)hipEventDestroy(e);
@@ -457,9 +499,12 @@ static atr_status plan_execute(Plan& p, const float* in, float* out, int accumul
         }
         launch_zero_int(p.d_sflag, p.vol_y, s);
         timed(p, K_WEIGHT, s, [&] { launch_filter_wbp(p, in, s); });
-        for (int y0 = 0; y0 < p.vol_y; y0 += p.wbp_slab) {
+        for (int y0 = 0, k = 0; y0 < p.vol_y; y0 += p.wbp_slab, k++) {
             int ny = std::min(p.wbp_slab, p.vol_y - y0);
-            timed(p, K_WBP_BP, s, [&] { launch_wbp_bp_rowfft(p, y0, ny, s); });
+            timed(p, K_WBP_BP, s, [&] {
+                launch_wbp_bp(p, k, ny, s);
+                launch_wbp_rowfft(p, ny, s);
+            });
             timed(p, K_WBP_COL, s, [&] { launch_wbp_colfft(p, y0, ny, s); });
             timed(p, K_WBP_ROW, s, [&] { launch_wbp_rowc2r(p, y0, ny, out, s); });
         }

@@ -136,6 +136,8 @@ struct Plan {
     size_t tab_bytes[9] = {0};        // sizes of the tile-shape dependent tables
     BpArgs* d_bp_args = nullptr;      // SART step arguments: initial BP, then per subset
     FpArgs* d_fp_args = nullptr;      // per subset
+    float* d_wbp_grid = nullptr;      // [slab][2VolZ][2VolX] the 2x-grid back-projection
+    int2* d_wbp_tiles = nullptr;      // [N][2VolZ/16][2VolX/256] window start, class
     float2* d_wbp_t = nullptr;        // [slab][2VolZ][VolX/2+1]
     float2* d_wbp_u = nullptr;        // [slab][VolZ][VolX/2+1]
     int wbp_slab = 0;
@@ -164,6 +166,7 @@ struct BpArgs {
     int n_ztiles, n_xtiles;
     float relax;
     int positivity;
+    float scale, offset;       // u = scale (fx cos + fz sin) + offset
     const int* sflag;          // NULL: the sinogram holds no empty samples
     float* vol;
     int vol_x, vol_z;
@@ -216,7 +219,7 @@ void cos_sin_table(const float* tilts, int n, std::vector<float>& cs);
 void build_bp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                         std::vector<int2>& lohi);
 void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi, int n, int vol_x,
-                    int vol_z, int proj, int tz, std::vector<int2>& tiles);
+                    int vol_z, double scale, double offset, int tz, std::vector<int2>& tiles);
 void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                          std::vector<int2>& lohi);
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
@@ -255,7 +258,9 @@ int batch_blocks_per_cu(int storage, bool fp, size_t fp_shm);
 void launch_fp_diff(const Plan& p, int subset, hipStream_t s);
 void launch_vol_stats(const Plan& p, hipStream_t s);
 void launch_output(const Plan& p, float* out, hipStream_t s);
-void launch_wbp_bp_rowfft(const Plan& p, int y0, int ny, hipStream_t s);
+// slab k of the WBP: back-projection onto the 2x grid (arguments p.d_bp_args + k)
+void launch_wbp_bp(const Plan& p, int slab, int ny, hipStream_t s);
+void launch_wbp_rowfft(const Plan& p, int ny, hipStream_t s);
 void launch_wbp_colfft(const Plan& p, int y0, int ny, hipStream_t s);
 void launch_wbp_rowc2r(const Plan& p, int y0, int ny, float* out, hipStream_t s);
 void launch_load_volume(const Plan& p, const float* in, hipStream_t s);

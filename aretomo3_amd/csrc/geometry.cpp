@@ -126,14 +126,16 @@ void build_bp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_
     });
 }
 
-// Per (tilt, z-tile, x-tile) of the SART back-projector: the start w0 of the
+// Per (tilt, z-tile, x-tile) of a back-projection grid: the start w0 of the
 // window of BP_TX + tz + 16 row samples the tile stages, and the tile class
 // (1 = every voxel of the tile uses the tilt, 0 = none does, 2 = some do).
-// w0 = floor(min u over the tile corners) - 3 with u = fx cos + fz sin + P/2
-// in FP64; the kernel's FP32 coordinates relative to w0 then stay in
-// [2, window - 2] (the tile spans at most (BP_TX-1)|cos| + (tz-1)|sin| in u).
+// The grid coordinate is u = scale (fx cos + fz sin) + offset (SART: scale 1,
+// offset P/2; the WBP 2x grid: scale 1/2, offset P/2 + 1/2).
+// w0 = floor(min u over the tile corners) - 3 in FP64; the kernel's FP32
+// coordinates relative to w0 then stay in [2, window - 2] (the tile spans at
+// most scale ((BP_TX-1)|cos| + (tz-1)|sin|) in u).
 void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi, int n, int vol_x,
-                    int vol_z, int proj, int tz, std::vector<int2>& tiles)
+                    int vol_z, double scale, double offset, int tz, std::vector<int2>& tiles)
 {
     int nzt = (vol_z + tz - 1) / tz, nxt = (vol_x + BP_TX - 1) / BP_TX;
     tiles.assign((size_t)n * nzt * nxt, int2{0, 0});
@@ -150,7 +152,8 @@ void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi,
                 }
                 const double fx0 = x0 - vol_x * 0.5, fx1 = x1 - vol_x * 0.5;
                 const double fz0 = zt * tz - vol_z * 0.5, fz1 = fz0 + (tz - 1);
-                const double umin = proj / 2.0 + std::min(c * fx0, c * fx1) + std::min(s * fz0, s * fz1);
+                const double umin =
+                    offset + scale * (std::min(c * fx0, c * fx1) + std::min(s * fz0, s * fz1));
                 tiles[((size_t)t * nzt + zt) * nxt + xt] =
                     int2{(int)std::floor(umin) - 3, full ? 1 : (any ? 2 : 0)};
             }

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_weight(const float* __restrict__ in, si
 // ------------------------------------------------------------------------
 // One tile: voxels x0 .. x0+BP_TX-1, z0 .. z0+TZ-1 of slice y.  Ends
 // after a barrier, with the LDS free for the next tile.
-template <typename S, int TZ, int CH>
+template <typename S, int TZ, int CH, bool WBP = false>
 __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_win(TZ)],
                                         int xt, int zt, int y)
 {
@@ -189,6 +189,7 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
     const int n_ztiles = A->n_ztiles, n_xtiles = A->n_xtiles;
     const float relax = A->relax;
     const int positivity = A->positivity;
+    const double scale = A->scale, offset = A->offset;
     const cptr<int> sflag = as_cptr(A->sflag);
     const gptr<float> vol = as_gptr(A->vol);
     const int vol_x = A->vol_x, vol_z = A->vol_z;
@@ -202,9 +203,11 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
     const gptr<float> vslice = vol + (size_t)y * vol_z * vol_x;
 
     // the voxels this thread updates, loaded now and consumed at the end
+    // (the WBP grid is written, not updated)
     float vold[TZ];
 #pragma unroll
-    for (int k = 0; k < TZ; k++) vold[k] = vslice[(size_t)min(z0 + k, vol_z - 1) * vol_x + xc];
+    for (int k = 0; k < TZ; k++)
+        vold[k] = WBP ? 0.0f : vslice[(size_t)min(z0 + k, vol_z - 1) * vol_x + xc];
 
     float acc[TZ];
     int cnt[TZ];
@@ -275,14 +278,15 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
             if (tw.y == 0) continue;
             const float2 sc = ldc(cs + tr.tilt);
             const float c = sc.x, s = sc.y;
-            const double kk = fz0 * (double)s + proj / 2.0 - (double)tw.x;
-            const float u0 = (float)(fx * (double)c + kk);
+            const double kk = scale * fz0 * (double)s + offset - (double)tw.x;
+            const float u0 = (float)(scale * fx * (double)c + kk);
+            const float ss = (float)(scale * (double)s);   // exact: scale is 1 or 1/2
             const float2* pj = pairs[j];
             if (tw.y == 1 && !sent) {
                 full_cnt += 1;
 #pragma unroll
                 for (int k = 0; k < TZ; k++) {
-                    float u = fmaf((float)k, s, u0);
+                    float u = fmaf((float)k, ss, u0);
                     int i = (int)u;
                     float f = __builtin_amdgcn_fractf(u);
                     float2 pr = pj[i];
@@ -293,7 +297,7 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
 #pragma unroll
                 for (int k = 0; k < TZ; k++) {
                     const int2 r = ldc(lh + min(z0 + k, vol_z - 1));
-                    float u = fmaf((float)k, s, u0);
+                    float u = fmaf((float)k, ss, u0);
                     int i = (int)u;
                     float f = __builtin_amdgcn_fractf(u);
                     float2 pr = pj[i];
@@ -305,6 +309,17 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
             }
         }
         __syncthreads();
+    }
+    if (WBP) {
+        // ---- mean over the tilts used, 0 when none (GBackProjWbp.cu:48-52) ----
+#pragma unroll
+        for (int k = 0; k < TZ; k++) {
+            const int z = z0 + k;
+            const int count = full_cnt + cnt[k];
+            const float v = count > 0 ? acc[k] * __builtin_amdgcn_rcpf((float)count) : 0.0f;
+            if (x < vol_x && z < vol_z) vslice[(size_t)z * vol_x + x] = v;
+        }
+        return;
     }
     // ---- v = relax * sum / count + v, positivity, sentinel (GBackProj.cu:50-59) ----
     bool wrote_sent = false;
@@ -328,6 +343,16 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj(const BpArgs* __restrict__ 
 {
     __shared__ float2 pairs[CH][bp_win(TZ)];
     bp_tile<S, TZ, CH>(as_cptr(a), pairs, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// WBP back-projection onto the 2x grid (Recon/GBackProjWbp.cu:14-53): the
+// same tile machinery, coordinate u = (fx cos + fz sin)/2 + P/2 + 1/2, the
+// mean written to the grid buffer.
+template <typename S>
+__global__ __launch_bounds__(BP_TX) void k_wbp_grid_bp(const BpArgs* __restrict__ a)
+{
+    __shared__ float2 pairs[BP_CHUNK][bp_win(BP_TZ)];
+    bp_tile<S, BP_TZ, BP_CHUNK, true>(as_cptr(a), pairs, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Block-wide work queue: thread 0 takes the next item index, every thread
@@ -774,6 +799,8 @@ BpArgs bp_args(const Plan& p, const void* sino, int rows, const TiltRow* list, i
     a.n_xtiles = (p.vol_x + BP_TX - 1) / BP_TX;
     a.relax = relax;
     a.positivity = p.positivity;
+    a.scale = 1.0f;
+    a.offset = (float)(p.proj / 2.0);
     a.sflag = check_sent ? p.d_sflag : nullptr;
     a.vol = p.d_vol;
     a.vol_x = p.vol_x;
@@ -806,6 +833,16 @@ void launch_bp(const Plan& p, int step, hipStream_t s)
     if (p.storage == ATR_STORAGE_F16) launch_bp_t<_Float16>(p, step, s);
     else if (p.storage == ATR_STORAGE_BF16) launch_bp_t<bf16_t>(p, step, s);
     else launch_bp_t<float>(p, step, s);
+}
+
+void launch_wbp_bp(const Plan& p, int slab, int ny, hipStream_t s)
+{
+    const int nxt = (2 * p.vol_x + BP_TX - 1) / BP_TX, nzt = (2 * p.vol_z + BP_TZ - 1) / BP_TZ;
+    dim3 grid(nxt, nzt, ny);
+    const BpArgs* a = p.d_bp_args + slab;
+    if (p.storage == ATR_STORAGE_F16) hipLaunchKernelGGL(k_wbp_grid_bp<_Float16>, grid, dim3(BP_TX), 0, s, a);
+    else if (p.storage == ATR_STORAGE_BF16) hipLaunchKernelGGL(k_wbp_grid_bp<bf16_t>, grid, dim3(BP_TX), 0, s, a);
+    else hipLaunchKernelGGL(k_wbp_grid_bp<float>, grid, dim3(BP_TX), 0, s, a);
 }
 
 void launch_bp_batch(int storage, const BpItem* items, int n_items, int total, int* counter,

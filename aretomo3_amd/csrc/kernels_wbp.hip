@@ -1,15 +1,16 @@
 // kernels_wbp.hip -- weighted back-projection path (gfx950):
 //   k_filter        weighting + ramp/Hamming filter, LDS row FFT
 //                   (Recon/GWeightProjs.cu:7-19, Recon/GRWeight.cu:7-69)
-//   k_wbp_bp_rowfft 2x-grid back-projection of two grid rows + their row
-//                   FFT, keeping the bins the Fourier crop uses
-//                   (Recon/GBackProjWbp.cu:14-53,126)
+//   k_wbp_rowfft    row FFT of two rows of the 2x grid (the back-projection
+//                   onto that grid is k_wbp_grid_bp in kernels_sart.hip)
+//                   (Recon/GBackProjWbp.cu:126)
 //   k_wbp_colfft    column FFT, normalisation, crop, inverse column FFT
 //                   (MaUtil/CCufft2D.cpp:65-80, MaUtil/GFourierResize2D.cu:7-39)
 //   k_wbp_rowc2r    real inverse row FFT + output layout
 //                   (MaUtil/CCufft2D.cpp:108-123, MaUtil/CPad2D.cpp:29-39,
 //                    Recon/CDoWbpRecon.cpp:107-123)
-// Scratch: d_wbp_t [slab][2VolZ][VolX/2+1], d_wbp_u [slab][VolZ][VolX/2+1].
+// Scratch: d_wbp_grid [slab][2VolZ][2VolX], d_wbp_t [slab][2VolZ][VolX/2+1],
+// d_wbp_u [slab][VolZ][VolX/2+1].
 #include "atr_internal.h"
 #include "storage.h"
 
@@ -103,63 +104,22 @@ __global__ __launch_bounds__(256) void k_filter(const float* __restrict__ in, si
 // mean of the linear interpolations of the tilts whose FP64 inclusion
 // interval holds it (0 when none).  The two rows are transformed together as
 // one complex row of length 2VolX; bins 0..VolX/2 of each are written.
-template <typename S>
-__global__ __launch_bounds__(256) void k_wbp_bp_rowfft(
-    const S* __restrict__ sino, int n, int stride, int proj, const float2* __restrict__ cs,
-    const unsigned char* __restrict__ excl, const int2* __restrict__ lohi,
-    const int* __restrict__ sflag, int vol_x, int vol_z, int y0, FftDesc fd,
-    float2* __restrict__ tbuf)
+// Row FFT of the 2x grid (Recon/GBackProjWbp.cu:126-130 forward R2C along x):
+// two grid rows packed as one complex row, one LDS FFT of length 2 VolX, the
+// two half spectra split apart.
+__global__ __launch_bounds__(256) void k_wbp_rowfft(const float* __restrict__ grid, int vol_x,
+                                                    int vol_z, FftDesc fd,
+                                                    float2* __restrict__ tbuf)
 {
     extern __shared__ float2 lds[];
     const int nx2 = 2 * vol_x, nz2 = 2 * vol_z, kxn = vol_x / 2 + 1;
     float2* a = lds;
     float2* b = lds + nx2;
-    const int zp = blockIdx.x, ys = blockIdx.y, y = y0 + ys;
+    const int zp = blockIdx.x, ys = blockIdx.y;
     const int za = 2 * zp, zb = za + 1;
-    const S* base = sino + (size_t)y * n * stride;
-    const bool sent = sflag[y] != 0;
-    const float thr = empty_threshold<S>();
-    const int cx = proj / 2;
-    for (int x = threadIdx.x; x < nx2; x += blockDim.x) {
-        const float fx = (float)((double)x - nx2 / 2.0);
-        float sa = 0.0f, sb = 0.0f;
-        int ca = 0, cb = 0;
-        for (int t = 0; t < n; t++) {
-            if (excl[t]) continue;
-            const float c = cs[t].x, s = cs[t].y;
-            const float hc = 0.5f * c;
-            const S* row = base + (size_t)t * stride;
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int z = h == 0 ? za : zb;
-                const int2 r = lohi[(size_t)t * nz2 + z];
-                if (x < r.x || x > r.y) continue;
-                const double fz = (double)z - nz2 / 2.0;
-                const float kk = (float)(fz * (double)s / 2 + cx + 0.5);
-                float u = fmaf(fx, hc, kk);
-                u = __builtin_amdgcn_fmed3f(u, 0.0f, (float)(proj - 2));
-                int iu = (int)u;
-                float f = __builtin_amdgcn_fractf(u);
-                float p0 = ld<S>(row + iu), p1 = ld<S>(row + iu + 1);
-                if (sent) {
-                    if (p0 <= thr) p0 = kSentinel;
-                    if (p1 <= thr) p1 = kSentinel;
-                }
-                float v = p0 * (1.0f - f) + p1 * f;
-                if (sent && v <= kEmpty) continue;
-                if (h == 0) {
-                    sa += v;
-                    ca++;
-                } else {
-                    sb += v;
-                    cb++;
-                }
-            }
-        }
-        float va = ca > 0 ? sa / (float)ca : 0.0f;
-        float vb = cb > 0 ? sb / (float)cb : 0.0f;
-        a[x] = make_float2(va, vb);
-    }
+    const float* ra = grid + ((size_t)ys * nz2 + za) * nx2;
+    const float* rb = ra + nx2;
+    for (int x = threadIdx.x; x < nx2; x += blockDim.x) a[x] = make_float2(ra[x], rb[x]);
     __syncthreads();
     float2* r = lds_fft(a, b, fd, 1, -1.0f, threadIdx.x, blockDim.x);
     float2* ta = tbuf + ((size_t)ys * nz2 + za) * kxn;
@@ -283,21 +243,12 @@ void launch_filter_wbp(const Plan& p, const float* in, hipStream_t s)
     else launch_filter_t<float>(p, in, s);
 }
 
-template <typename S>
-static void launch_wbp_bp_t(const Plan& p, int y0, int ny, hipStream_t s)
+void launch_wbp_rowfft(const Plan& p, int ny, hipStream_t s)
 {
     dim3 grid(p.vol_z, ny);
     size_t shm = 2 * (size_t)(2 * p.vol_x) * sizeof(float2);
-    hipLaunchKernelGGL(k_wbp_bp_rowfft<S>, grid, dim3(256), shm, s, (const S*)p.d_sino, p.n,
-                       p.sino_stride, p.proj, p.d_cs, p.d_excl, p.d_wbp_lohi, p.d_sflag, p.vol_x,
-                       p.vol_z, y0, p.fft_x2, p.d_wbp_t);
-}
-
-void launch_wbp_bp_rowfft(const Plan& p, int y0, int ny, hipStream_t s)
-{
-    if (p.storage == ATR_STORAGE_F16) launch_wbp_bp_t<_Float16>(p, y0, ny, s);
-    else if (p.storage == ATR_STORAGE_BF16) launch_wbp_bp_t<bf16_t>(p, y0, ny, s);
-    else launch_wbp_bp_t<float>(p, y0, ny, s);
+    hipLaunchKernelGGL(k_wbp_rowfft, grid, dim3(256), shm, s, p.d_wbp_grid, p.vol_x, p.vol_z,
+                       p.fft_x2, p.d_wbp_t);
 }
 
 int wbp_col_block(int vol_z)
