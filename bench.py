@@ -95,6 +95,26 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
 
 
+def fourier_ring_correlation(a, b, n_rings=32):
+    """Mean over slices of the Fourier ring correlation of two stacks of XZ
+    slices [k][z][x], per ring of normalised radius; returns the curve."""
+    fa = np.fft.rfft2(a.astype(np.float64))
+    fb = np.fft.rfft2(b.astype(np.float64))
+    nz, nx = a.shape[1], a.shape[2]
+    kz = np.fft.fftfreq(nz)[:, None]
+    kx = np.fft.rfftfreq(nx)[None, :]
+    r = np.sqrt(kz ** 2 + kx ** 2) / 0.5
+    ring = np.minimum((r * n_rings).astype(int), n_rings)
+    num = np.zeros(n_rings + 1)
+    da = np.zeros(n_rings + 1)
+    db = np.zeros(n_rings + 1)
+    for k in range(a.shape[0]):
+        num += np.bincount(ring.ravel(), (fa[k] * np.conj(fb[k])).real.ravel(), n_rings + 1)
+        da += np.bincount(ring.ravel(), (np.abs(fa[k]) ** 2).ravel(), n_rings + 1)
+        db += np.bincount(ring.ravel(), (np.abs(fb[k]) ** 2).ravel(), n_rings + 1)
+    return (num / np.sqrt(np.maximum(da * db, 1e-300)))[:n_rings]
+
+
 def describe(name, w, vol):
     vx, vy, vz = vol
     n = int(round(120.0 / w["step"])) + 1
@@ -432,6 +452,10 @@ def main():
                       (len(slices), vol_y, "SART" if sart else "WBP", t_pre, t_cpu),
         }
         result["rel_l2_vs_fp64"] = rel
+        frc = fourier_ring_correlation(got, ref)
+        result["frc_vs_fp64"] = {"min_to_nyquist": round(float(frc.min()), 6),
+                                 "at_half_nyquist": round(float(frc[len(frc) // 2]), 6),
+                                 "rings": len(frc)}
     print(json.dumps(result))
 
 
