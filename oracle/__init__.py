@@ -55,15 +55,19 @@ def lib():
             "orc_back_proj_wbp": (None, [_dp, _c_int, _c_int, _fp, _c_vp, _c_int, _dp,
                                          _c_int, _c_int]),
             "orc_wbp_crop": (None, [_dp, _c_int, _c_int, _dp]),
+            "orc_fourier_crop": (None, [_dp, _c_int, _c_int, _c_int, _c_int, _dp]),
+            "orc_bin_size": (None, [_c_int, _c_int, _c_f, ctypes.POINTER(_c_int),
+                                    ctypes.POINTER(_c_int)]),
+            "orc_bin_stack": (None, [_dp, _c_int, _c_int, _c_int, _c_int, _c_int, _dp]),
             "orc_tomo_sart": (None, [_dp, _c_int, _c_int, _c_int, _fp, _c_vp, _dp, _c_int,
                                      _c_int, _c_int, _c_int, _c_int, _c_int, _c_d, _c_int,
                                      _c_int, _ip]),
             "orc_tomo_wbp": (None, [_dp, _c_int, _c_int, _c_int, _fp, _c_vp, _dp, _c_int,
                                     _c_int]),
-            "orc_corr_proj": (None, [_fp, _c_int, _c_int, _c_f, _c_f, _c_f, _dp, _c_int,
-                                     _c_int, _c_int, _c_int, _c_vp]),
-            "orc_prestage": (None, [_fp, _c_int, _c_int, _c_int, _fp, _c_vp, _c_int, _c_u64,
-                                    _dp, _c_int, _c_int]),
+            "orc_corr_proj": (None, [_fp, _c_int, _c_int, _c_f, _c_f, _c_f, _c_int, _c_vp,
+                                     _dp, _c_int, _c_int, _c_int, _c_int, _c_vp]),
+            "orc_prestage": (None, [_fp, _c_int, _c_int, _c_int, _fp, _c_vp, _c_int, _c_vp,
+                                    _c_int, _c_u64, _dp, _c_int, _c_int]),
             "orc_recon_slices": (None, [_dp, _c_int, _c_int, _c_int, _fp, _c_vp, _c_int,
                                         _c_int, _c_int, _c_int, _c_int, _c_int, _c_d, _c_int,
                                         _c_int, _c_u64, _ip, _c_int, _dp]),
@@ -198,6 +202,30 @@ def wbp_crop(img2, vol_x, vol_z):
     return out
 
 
+def fourier_crop(img, bx, by):
+    img = _f64(img)
+    ay, ax = img.shape
+    out = np.empty((by, bx), np.float64)
+    lib().orc_fourier_crop(img, ax, ay, bx, by, out)
+    return out
+
+
+def bin_size(ax, ay, bin_):
+    bx, by = _c_int(), _c_int()
+    lib().orc_bin_size(ax, ay, bin_, ctypes.byref(bx), ctypes.byref(by))
+    return bx.value, by.value
+
+
+def bin_stack(aligned, bin_):
+    """Fourier binning of an aligned stack [n][ay][ax] (CBinStack::DoFFT)."""
+    a = _f64(aligned)
+    n, ay, ax = a.shape
+    bx, by = bin_size(ax, ay, float(bin_))
+    out = np.empty((n, by, bx), np.float64)
+    lib().orc_bin_stack(a, n, ax, ay, bx, by, out)
+    return out
+
+
 def tomo_sart(sino, proj_x, cs, vol, start_tilt, n_sart_tilts, iters, n_subsets,
               relax=-1.0, order=0, positivity=1, fill_tbl=None, no_proj=None):
     s = _f64(sino).copy()
@@ -219,20 +247,27 @@ def tomo_wbp(sino, proj_x, cs, vol_x, vol_z, no_proj=None):
     return v, s
 
 
-def corr_proj(raw, axis, shift, out_x, out_y, out_pad_x, random_fill=0, table=None):
+def corr_proj(raw, axis, shift, out_x, out_y, out_pad_x, random_fill=0, table=None,
+              local=None):
+    """local: patch table of this tilt, [5][n_patches] (cx, cy, sx, sy, good)."""
     raw = _f32(raw)
     out = np.zeros((out_y, out_pad_x), np.float64)
     tp = None
     if table is not None:
         table = _i32(table)
         tp = table.ctypes.data_as(ctypes.c_void_p)
-    lib().orc_corr_proj(raw, raw.shape[1], raw.shape[0], axis, shift[0], shift[1], out,
-                        out_x, out_y, out_pad_x, random_fill, tp)
+    npat, lp = 0, None
+    if local is not None:
+        local = _f32(local).reshape(5, -1)
+        npat, lp = local.shape[1], local.ctypes.data_as(ctypes.c_void_p)
+    lib().orc_corr_proj(raw, raw.shape[1], raw.shape[0], axis, shift[0], shift[1], npat, lp,
+                        out, out_x, out_y, out_pad_x, random_fill, tp)
     return out
 
 
-def prestage(raw, axes, shifts, random_fill=1, seed=0):
-    """raw: [n][raw_y][raw_x] float32 -> aligned [n][aln_y][aln_x] float64."""
+def prestage(raw, axes, shifts, random_fill=1, seed=0, local=None):
+    """raw: [n][raw_y][raw_x] float32 -> aligned [n][aln_y][aln_x] float64.
+    local: per-tilt patch tables [n][5][n_patches] (cx, cy, sx, sy, good)."""
     raw = _f32(raw)
     n, ry, rx = raw.shape
     ax, ay = aligned_size(rx, ry, float(np.float32(axes[0])))
@@ -240,9 +275,13 @@ def prestage(raw, axes, shifts, random_fill=1, seed=0):
     sh = None
     if shifts is not None:
         sh = _f32(shifts)
+    npat, lp = 0, None
+    if local is not None:
+        local = _f32(local).reshape(n, 5, -1)
+        npat, lp = local.shape[2], local.ctypes.data_as(ctypes.c_void_p)
     lib().orc_prestage(raw, n, rx, ry, _f32(axes),
                        None if sh is None else sh.ctypes.data_as(ctypes.c_void_p),
-                       random_fill, seed, out, ax, ay)
+                       npat, lp, random_fill, seed, out, ax, ay)
     return out
 
 

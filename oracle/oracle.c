@@ -429,81 +429,109 @@ void orc_back_proj_wbp(const double* sino, int pad_x, int proj_x,
     }
 }
 
-/* Recon/GBackProjWbp.cu:126-141 with MaUtil/CCufft2D.cpp:65-80 (forward R2C,
- * x 1/(2VolX*2VolZ)), MaUtil/GFourierResize2D.cu:7-39 (crop: output row
- * j <= VolZ/2 from input row j, j > VolZ/2 from input row j + VolZ; columns
- * 0..VolX/2) and MaUtil/CCufft2D.cpp:108-123 (unnormalised C2R VolZ x VolX).
- * The crop leaves the Nyquist row/column non-Hermitian; the C2R is defined
- * here as a complex inverse along z followed by a real inverse along x that
- * ignores the imaginary parts of the DC and Nyquist bins (numpy irfft2). */
-void orc_wbp_crop(const double* img2, int vol_x, int vol_z, double* vol)
+/* Fourier crop of a real image ax x ay to bx x by (bx <= ax, by <= ay, all
+ * even): forward R2C normalised by 1/(ax*ay) (MaUtil/CCufft2D.cpp:65-80),
+ * crop (MaUtil/GFourierResize2D.cu:7-39: output row j <= by/2 from input row
+ * j, j > by/2 from input row j - by + ay; columns 0..bx/2), unnormalised
+ * C2R (MaUtil/CCufft2D.cpp:108-123).  The crop leaves the Nyquist row and
+ * column non-Hermitian; the C2R is defined here as a complex inverse along y
+ * followed by a real inverse along x that ignores the imaginary parts of the
+ * DC and Nyquist bins (numpy irfft2). */
+void orc_fourier_crop(const double* img, int ax, int ay, int bx, int by, double* out)
 {
-    int nx2 = 2 * vol_x, nz2 = 2 * vol_z, kx_n = vol_x / 2 + 1;
-    double* ar = (double*)malloc(sizeof(double) * (size_t)nz2 * kx_n);
-    double* ai = (double*)malloc(sizeof(double) * (size_t)nz2 * kx_n);
-    int big = nx2 > nz2 ? nx2 : nz2;
+    int kx_n = bx / 2 + 1;
+    double* ar = (double*)malloc(sizeof(double) * (size_t)ay * kx_n);
+    double* ai = (double*)malloc(sizeof(double) * (size_t)ay * kx_n);
+    int big = ax > ay ? ax : ay;
     double* re = (double*)malloc(sizeof(double) * big);
     double* im = (double*)malloc(sizeof(double) * big);
-    /* forward along x (rows of the 2x grid) */
-    for (int z = 0; z < nz2; z++) {
-        for (int x = 0; x < nx2; x++) {
-            re[x] = img2[(size_t)z * nx2 + x];
+    /* forward along x */
+    for (int z = 0; z < ay; z++) {
+        for (int x = 0; x < ax; x++) {
+            re[x] = img[(size_t)z * ax + x];
             im[x] = 0.0;
         }
-        orc_fft(re, im, nx2, -1);
+        orc_fft(re, im, ax, -1);
         for (int k = 0; k < kx_n; k++) {
             ar[(size_t)z * kx_n + k] = re[k];
             ai[(size_t)z * kx_n + k] = im[k];
         }
     }
-    double norm = 1.0 / ((double)nx2 * (double)nz2);
-    double* hr = (double*)malloc(sizeof(double) * (size_t)vol_z * kx_n);
-    double* hi = (double*)malloc(sizeof(double) * (size_t)vol_z * kx_n);
-    /* forward along z, normalise, crop, inverse along z */
+    double norm = 1.0 / ((double)ax * (double)ay);
+    double* hr = (double*)malloc(sizeof(double) * (size_t)by * kx_n);
+    double* hi = (double*)malloc(sizeof(double) * (size_t)by * kx_n);
+    double* gr = (double*)malloc(sizeof(double) * by);
+    double* gi = (double*)malloc(sizeof(double) * by);
+    /* forward along y, normalise, crop, inverse along y */
     for (int k = 0; k < kx_n; k++) {
-        for (int z = 0; z < nz2; z++) {
+        for (int z = 0; z < ay; z++) {
             re[z] = ar[(size_t)z * kx_n + k];
             im[z] = ai[(size_t)z * kx_n + k];
         }
-        orc_fft(re, im, nz2, -1);
-        double* gr = (double*)malloc(sizeof(double) * vol_z);
-        double* gi = (double*)malloc(sizeof(double) * vol_z);
-        for (int j = 0; j < vol_z; j++) {
-            int src = j <= vol_z / 2 ? j : j + vol_z;
+        orc_fft(re, im, ay, -1);
+        for (int j = 0; j < by; j++) {
+            int src = j <= by / 2 ? j : j - by + ay;
             gr[j] = re[src] * norm;
             gi[j] = im[src] * norm;
         }
-        orc_fft(gr, gi, vol_z, +1);
-        for (int z = 0; z < vol_z; z++) {
+        orc_fft(gr, gi, by, +1);
+        for (int z = 0; z < by; z++) {
             hr[(size_t)z * kx_n + k] = gr[z];
             hi[(size_t)z * kx_n + k] = gi[z];
         }
-        free(gr);
-        free(gi);
     }
     /* real inverse along x */
-    for (int z = 0; z < vol_z; z++) {
+    for (int z = 0; z < by; z++) {
         const double* r = hr + (size_t)z * kx_n;
         const double* i = hi + (size_t)z * kx_n;
         re[0] = r[0];
         im[0] = 0.0;
-        re[vol_x / 2] = r[vol_x / 2];
-        im[vol_x / 2] = 0.0;
-        for (int k = 1; k < vol_x / 2; k++) {
+        re[bx / 2] = r[bx / 2];
+        im[bx / 2] = 0.0;
+        for (int k = 1; k < bx / 2; k++) {
             re[k] = r[k];
             im[k] = i[k];
-            re[vol_x - k] = r[k];
-            im[vol_x - k] = -i[k];
+            re[bx - k] = r[k];
+            im[bx - k] = -i[k];
         }
-        orc_fft(re, im, vol_x, +1);
-        for (int x = 0; x < vol_x; x++) vol[(size_t)z * vol_x + x] = re[x];
+        orc_fft(re, im, bx, +1);
+        for (int x = 0; x < bx; x++) out[(size_t)z * bx + x] = re[x];
     }
     free(ar);
     free(ai);
     free(hr);
     free(hi);
+    free(gr);
+    free(gi);
     free(re);
     free(im);
+}
+
+/* Recon/GBackProjWbp.cu:126-141: the 2x grid (2 VolX x 2 VolZ) cropped to
+ * VolX x VolZ (orc_fourier_crop). */
+void orc_wbp_crop(const double* img2, int vol_x, int vol_z, double* vol)
+{
+    orc_fourier_crop(img2, 2 * vol_x, 2 * vol_z, vol_x, vol_z, vol);
+}
+
+/* Correct/CBinStack.cpp:54-106 (-AtBin > 1): every image of the aligned stack
+ * Fourier-cropped to the binned size of MaUtil/GFourierResize2D.cu:24-38,
+ * ((int)(size / bin)) / 2 * 2 in each direction.  in [n][ay][ax] ->
+ * out [n][by][bx]. */
+void orc_bin_size(int ax, int ay, float bin, int* bx, int* by)
+{
+    *bx = ax;
+    *by = ay;
+    if (bin == 1.0f) return;
+    *bx = (int)(ax / bin) / 2 * 2;
+    *by = (int)(ay / bin) / 2 * 2;
+}
+
+void orc_bin_stack(const double* in, int n, int ax, int ay, int bx, int by, double* out)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < n; t++)
+        orc_fourier_crop(in + (size_t)t * ax * ay, ax, ay, bx, by, out + (size_t)t * bx * by);
 }
 
 void orc_tomo_wbp(double* sino, int n_tilts, int pad_x, int proj_x,
@@ -520,11 +548,36 @@ void orc_tomo_wbp(double* sino, int n_tilts, int pad_x, int proj_x,
 }
 
 /* ---------------------------------------------------------------------- */
-/* Pre-stage: Correct/GCorrPatchShift.cu:58-100 (global shift + tilt-axis   */
-/* rotation, nearest neighbour) and :134-154 (random fill)                 */
+/* Pre-stage: Correct/GCorrPatchShift.cu:58-100 (global + local shift,      */
+/* tilt-axis rotation, nearest neighbour) and :134-154 (random fill)       */
 /* ---------------------------------------------------------------------- */
+/* Correct/GCorrPatchShift.cu:18-48: local shift at rotated, centred (x, y)
+ * from the patch table of one tilt (cx[n], cy[n], sx[n], sy[n], good[n]):
+ * average of the good patches' shifts (good >= 0.9) weighted by
+ * exp(-100 ((x - cx)/raw_x)^2 + ((y - cy)/raw_y)^2), 0 if no good patch. */
+static void orc_local_shift(const float* lp, int np, double x, double y,
+                            int raw_x, int raw_y, double* lsx, double* lsy)
+{
+    const float *cx = lp, *cy = lp + np, *sx = lp + 2 * np, *sy = lp + 3 * np;
+    const float* good = lp + 4 * np;
+    double wx = 0.0, wy = 0.0, sw = 0.0;
+    int count = 0;
+    for (int p = 0; p < np; p++) {
+        if (good[p] < 0.9f) continue;
+        double dx = (x - cx[p]) / raw_x, dy = (y - cy[p]) / raw_y;
+        double w = exp(-100.0 * (dx * dx + dy * dy));
+        wx += sx[p] * w;
+        wy += sy[p] * w;
+        sw += w;
+        count += 1;
+    }
+    *lsx = (count > 0 && sw > 0) ? wx / sw : 0.0;
+    *lsy = (count > 0 && sw > 0) ? wy / sw : 0.0;
+}
+
 void orc_corr_proj(const float* raw, int raw_x, int raw_y, float tilt_axis_deg,
-                   float shift_x, float shift_y, double* out, int out_x,
+                   float shift_x, float shift_y, int n_patches,
+                   const float* local_params, double* out, int out_x,
                    int out_y, int out_pad_x, int random_fill,
                    const int* fill_table)
 {
@@ -537,6 +590,12 @@ void orc_corr_proj(const float* raw, int raw_x, int raw_y, float tilt_axis_deg,
             double yc = y + 0.5 - out_y * 0.5;
             double xr = xc * ca - yc * sa;
             double yr = xc * sa + yc * ca;
+            if (n_patches > 0 && local_params) {
+                double lsx, lsy;
+                orc_local_shift(local_params, n_patches, xr, yr, raw_x, raw_y, &lsx, &lsy);
+                xr += lsx;
+                yr += lsy;
+            }
             xr += (shift_x + raw_x * 0.5);
             yr += (shift_y + raw_y * 0.5);
             int ix = (int)(xr + 0.5);
@@ -555,6 +614,7 @@ void orc_corr_proj(const float* raw, int raw_x, int raw_y, float tilt_axis_deg,
  * (AreTomo/CAreTomoMain.cpp:612-615).  Output [t][aln_y][aln_x]. */
 void orc_prestage(const float* raw, int n_tilts, int raw_x, int raw_y,
                   const float* tilt_axis_deg, const float* shifts_xy,
+                  int n_patches, const float* local_params,
                   int random_fill, uint64_t fill_seed, double* aligned,
                   int aln_x, int aln_y)
 {
@@ -566,9 +626,11 @@ void orc_prestage(const float* raw, int n_tilts, int raw_x, int raw_y,
         double* buf = (double*)malloc(sizeof(double) * (size_t)pad * aln_y);
         float sx = shifts_xy ? shifts_xy[2 * t] : 0.0f;
         float sy = shifts_xy ? shifts_xy[2 * t + 1] : 0.0f;
+        const float* lp = (n_patches > 0 && local_params)
+                              ? local_params + (size_t)t * 5 * n_patches : 0;
         orc_corr_proj(raw + (size_t)t * raw_x * raw_y, raw_x, raw_y,
-                      tilt_axis_deg[t], sx, sy, buf, aln_x, aln_y, pad,
-                      random_fill, table);
+                      tilt_axis_deg[t], sx, sy, lp ? n_patches : 0, lp, buf,
+                      aln_x, aln_y, pad, random_fill, table);
         for (int y = 0; y < aln_y; y++)
             memcpy(aligned + ((size_t)t * aln_y + y) * aln_x, buf + (size_t)y * pad,
                    sizeof(double) * aln_x);

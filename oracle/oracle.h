@@ -55,6 +55,7 @@ void orc_back_proj_wbp(const double* sino, int pad_x, int proj_x,
                        const float* cos_sin, const unsigned char* no_proj,
                        int n_tilts, double* img2, int vol_x2, int vol_z2);
 void orc_wbp_crop(const double* img2, int vol_x, int vol_z, double* vol);
+void orc_fourier_crop(const double* img, int ax, int ay, int bx, int by, double* out);
 
 /* ---- per-slice operators (CTomoSart / CTomoWbp) ------------------------ */
 void orc_tomo_sart(double* sino, int n_tilts, int pad_x, int proj_x,
@@ -68,13 +69,17 @@ void orc_tomo_wbp(double* sino, int n_tilts, int pad_x, int proj_x,
 
 /* ---- pre-stage and per-series drivers --------------------------------- */
 void orc_corr_proj(const float* raw, int raw_x, int raw_y, float tilt_axis_deg,
-                   float shift_x, float shift_y, double* out, int out_x,
+                   float shift_x, float shift_y, int n_patches,
+                   const float* local_params, double* out, int out_x,
                    int out_y, int out_pad_x, int random_fill,
                    const int* fill_table);
 void orc_prestage(const float* raw, int n_tilts, int raw_x, int raw_y,
                   const float* tilt_axis_deg, const float* shifts_xy,
+                  int n_patches, const float* local_params,
                   int random_fill, uint64_t fill_seed, double* aligned,
                   int aln_x, int aln_y);
+void orc_bin_size(int ax, int ay, float bin, int* bx, int* by);
+void orc_bin_stack(const double* in, int n, int ax, int ay, int bx, int by, double* out);
 void orc_recon_slices(const double* aligned, int n_tilts, int proj_x,
                       int proj_y, const float* tilts_deg,
                       const unsigned char* no_proj, int method, int vol_z,

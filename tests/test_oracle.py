@@ -536,3 +536,81 @@ def test_flip_vol_indices():
         for z in range(vz):
             np.testing.assert_array_equal(plain[z, y], xzy[y, z])
             np.testing.assert_array_equal(flipped[vz - 1 - z, y], xzy[y, z])
+
+
+# ---------------------------------------------------------------------------
+# local patch shifts and Fourier binning (pre-stage extensions)
+# ---------------------------------------------------------------------------
+def _patches(cx, cy, sx, sy, good):
+    return np.array([cx, cy, sx, sy, good], np.float32)
+
+
+def test_local_shift_uniform_patches_equal_a_global_shift():
+    # every good patch shifted alike: the weighted mean is that shift, whatever
+    # the weights, so the result is the global-shift-only image with shift + s
+    rx, ry = 24, 18
+    rng = np.random.default_rng(21)
+    raw = rng.random((ry, rx)).astype(np.float32)
+    cx, cy = [-6.0, 0.0, 7.0, 3.0], [-4.0, 5.0, 2.0, -7.0]
+    loc = _patches(cx, cy, [2, 2, 2, 2], [-1, -1, -1, -1], [1, 1, 1, 1])
+    a = oracle.corr_proj(raw, 30.0, (0.5, 1.0), rx, ry, rx, local=loc)
+    b = oracle.corr_proj(raw, 30.0, (2.5, 0.0), rx, ry, rx)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_local_shift_bad_patches_and_single_patch():
+    rx, ry = 20, 16
+    raw = (np.arange(rx * ry, dtype=np.float32) + 1).reshape(ry, rx)
+    # only patch 1 is good: its shift applies everywhere
+    loc = _patches([0.0, 5.0], [0.0, -3.0], [7.0, 3.0], [7.0, -2.0], [0.0, 1.0])
+    a = oracle.corr_proj(raw, 0.0, (0.0, 0.0), rx, ry, rx, local=loc)
+    b = oracle.corr_proj(raw, 0.0, (3.0, -2.0), rx, ry, rx)
+    np.testing.assert_array_equal(a, b)
+    # no good patch: no local shift (good < 0.9 counts as bad)
+    loc0 = _patches([0.0], [0.0], [5.0], [5.0], [0.89])
+    np.testing.assert_array_equal(oracle.corr_proj(raw, 0.0, (0.0, 0.0), rx, ry, rx, local=loc0),
+                                  oracle.corr_proj(raw, 0.0, (0.0, 0.0), rx, ry, rx))
+
+
+def test_local_shift_two_patch_weights():
+    # axis 0, pixel centre (xc, yc) = (x + 0.5 - W/2, y + 0.5 - H/2); patches
+    # at (-5, 0) with shift +4 px and (5, 0) with shift 0: at xc = -5 the
+    # weights are 1 and exp(-100 (10/W)^2), so the shift is 4 / (1 + e)
+    rx, ry = 40, 10
+    raw = (np.arange(rx * ry, dtype=np.float32)).reshape(ry, rx) * 0 + \
+        np.arange(rx, dtype=np.float32)[None, :]
+    loc = _patches([-5.0, 5.0], [0.0, 0.0], [4.0, 0.0], [0.0, 0.0], [1.0, 1.0])
+    out = oracle.corr_proj(raw, 0.0, (0.0, 0.0), rx, ry, rx, local=loc)
+    x = 14                       # xc = 14.5 - 20 = -5.5
+    xc = x + 0.5 - rx / 2
+    w1 = np.exp(-100 * ((xc + 5) / rx) ** 2)
+    w2 = np.exp(-100 * ((xc - 5) / rx) ** 2)
+    ls = 4 * w1 / (w1 + w2)
+    ix = int(xc + ls + rx / 2 + 0.5)
+    assert out[3, x] == raw[3, ix]
+    assert ix == 18
+
+
+@pytest.mark.parametrize("ax,ay,bx,by", [(32, 24, 16, 12), (30, 40, 10, 20), (64, 64, 24, 32)])
+def test_fourier_crop_vs_numpy(ax, ay, bx, by):
+    rng = np.random.default_rng(ax + ay)
+    img = rng.standard_normal((ay, ax))
+    f = np.fft.rfft2(img) / (ax * ay)
+    rows = [j if j <= by // 2 else j - by + ay for j in range(by)]
+    want = np.fft.irfft2(f[rows][:, :bx // 2 + 1], s=(by, bx)) * (bx * by)
+    np.testing.assert_allclose(oracle.fourier_crop(img, bx, by), want, atol=1e-12)
+
+
+def test_bin_stack_sizes_and_dc():
+    assert oracle.bin_size(4096, 4096, 4.0) == (1024, 1024)
+    assert oracle.bin_size(1000, 998, 3.0) == (332, 332)   # (int)(x / 3) / 2 * 2
+    assert oracle.bin_size(100, 80, 1.0) == (100, 80)
+    # a constant image keeps its value (normalised forward, plain inverse)
+    st = np.full((2, 24, 36), 2.5)
+    np.testing.assert_allclose(oracle.bin_stack(st, 2.0), 2.5, atol=1e-12)
+    # a cosine below the new Nyquist survives binning unchanged in amplitude
+    x = np.arange(36)
+    img = np.tile(np.cos(2 * np.pi * 3 * x / 36), (24, 1))[None]
+    out = oracle.bin_stack(img, 2.0)[0]
+    xb = np.arange(18)
+    np.testing.assert_allclose(out, np.tile(np.cos(2 * np.pi * 3 * xb / 18), (12, 1)), atol=1e-12)
