@@ -66,6 +66,9 @@ class SeriesDesc(ctypes.Structure):
         ("storage", ctypes.c_int),
         ("out_layout", ctypes.c_int),
         ("fill_seed", ctypes.c_uint64),
+        ("n_patches", ctypes.c_int),
+        ("local_params", ctypes.c_void_p),
+        ("bin", ctypes.c_float),
     ]
 
 

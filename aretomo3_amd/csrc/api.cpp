@@ -96,11 +96,42 @@ static atr_status plan_sizes(const atr_series_desc* d, Plan& p)
         p.aln_x = p.in_x;
         p.aln_y = p.in_y;
     }
-    p.fill = p.prestage && d->random_fill != 0;
+    p.n_patches = 0;
+    if (d->n_patches < 0 || d->n_patches > 2048)
+        return fail(ATR_ERR_INVALID_ARG, "n_patches must be in [0, 2048]");
+    if (d->n_patches > 0) {
+        if (!p.prestage || d->local_params == nullptr)
+            return fail(ATR_ERR_INVALID_ARG, "local patch shifts need tilt_axis_deg and local_params");
+        for (size_t i = 0; i < (size_t)p.n * 5 * d->n_patches; i++)
+            if (!std::isfinite(d->local_params[i]))
+                return fail(ATR_ERR_INVALID_ARG, "non-finite local alignment parameter");
+        p.n_patches = d->n_patches;
+    }
+    // -AtBin: CBinStack Fourier binning to ((int)(size / bin)) / 2 * 2
+    // (MaUtil/GFourierResize2D.cu:24-38), fill forced on (CCorrTomoStack.cpp:107)
+    const float bin = d->bin;
+    if (!(bin >= 1.0f) || !std::isfinite(bin)) return fail(ATR_ERR_INVALID_ARG, "bin must be >= 1");
+    p.binned = bin != 1.0f;
+    p.bin_x = p.aln_x;
+    p.bin_y = p.aln_y;
+    if (p.binned) {
+        p.bin_x = (int)(p.aln_x / bin) / 2 * 2;
+        p.bin_y = (int)(p.aln_y / bin) / 2 * 2;
+        if (p.bin_x < 4 || p.bin_y < 2)
+            return fail(ATR_ERR_INVALID_ARG, "binned image smaller than 4 x 2");
+        FftDesc f;
+        if (!make_fft(p.aln_x, f) || !make_fft(p.aln_y, f) || !make_fft(p.bin_x, f) ||
+            !make_fft(p.bin_y, f))
+            return fail(ATR_ERR_UNSUPPORTED_SIZE,
+                        "Fourier binning: image size has no FFT plan");
+        if (p.aln_x > 8192 || p.aln_y > 8192)
+            return fail(ATR_ERR_UNSUPPORTED_SIZE, "Fourier binning supports images up to 8192");
+    }
+    p.fill = p.prestage && (d->random_fill != 0 || p.binned);
     // CDoWbpRecon.cpp:37-38, GBackProj.cu:74: VolX = ProjX/2*2 = effective ProjX
-    p.proj = p.aln_x / 2 * 2;
+    p.proj = p.bin_x / 2 * 2;
     p.vol_x = p.proj;
-    p.vol_y = p.aln_y;
+    p.vol_y = p.bin_y;
     p.vol_z = d->vol_z;
     p.sino_stride = (p.proj + 63) / 64 * 64;
     p.method = d->method;
@@ -158,7 +189,7 @@ static atr_status plan_sizes(const atr_series_desc* d, Plan& p)
         if (!make_fft(p.proj, f) || !make_fft(2 * p.vol_x, f) || !make_fft(2 * p.vol_z, f) ||
             !make_fft(p.vol_z, f) || !make_fft(p.vol_x, f))
             return fail(ATR_ERR_UNSUPPORTED_SIZE,
-                        "WBP FFT sizes must factor into 2, 3 and 5 (ProjX, VolX, VolZ)");
+                        "WBP: ProjX, VolX or VolZ has no FFT plan");
         if (p.proj > 4096 || p.vol_x > 2048 || p.vol_z > 2048)
             return fail(ATR_ERR_UNSUPPORTED_SIZE, "WBP supports ProjX <= 4096, VolZ <= 2048");
         // slab of slices whose spectra are buffered between the WBP kernels
@@ -177,6 +208,12 @@ static size_t plan_bytes_estimate(const Plan& p)
     b += n * 64 + (size_t)p.vol_y * 64;
     if (p.prestage) {
         b += n * p.aln_x * (size_t)p.aln_y * 4 + n * p.aln_y * 16 + (size_t)p.aln_x * p.aln_y * 4;
+        b += n * 5 * (size_t)p.n_patches * 4;
+    }
+    if (p.binned) {
+        size_t kx = p.bin_x / 2 + 1;
+        b += n * (size_t)p.bin_x * p.bin_y * 4 + n * kx * (p.aln_y + p.bin_y) * 8;
+        b += (size_t)(p.aln_x + p.aln_y + p.bin_x + p.bin_y) * 8;
     }
     b += (size_t)p.vol_y * n * p.sino_stride * sb;
     if (p.method == ATR_METHOD_SART) {
@@ -292,6 +329,23 @@ static atr_status plan_build(Plan& p)
             fill_table(d.fill_seed, p.aln_x * p.aln_y, tbl);
             if ((st = upload(&p.d_fill_aln, tbl, B)) != ATR_OK) return st;
         }
+        if (p.n_patches > 0) {
+            std::vector<float> lp(d.local_params, d.local_params + (size_t)p.n * 5 * p.n_patches);
+            if ((st = upload(&p.d_local, lp, B)) != ATR_OK) return st;
+        }
+    }
+    if (p.binned) {
+        const int kx = p.bin_x / 2 + 1;
+        if ((st = dalloc((void**)&p.d_binned, sizeof(float) * p.n * (size_t)p.bin_x * p.bin_y, B)) != ATR_OK) return st;
+        if ((st = dalloc((void**)&p.d_bin_t, sizeof(float2) * p.n * (size_t)kx * p.aln_y, B)) != ATR_OK) return st;
+        if ((st = dalloc((void**)&p.d_bin_u, sizeof(float2) * p.n * (size_t)kx * p.bin_y, B)) != ATR_OK) return st;
+        FftDesc* fds[4] = {&p.fft_bin_ax, &p.fft_bin_ay, &p.fft_bin_by, &p.fft_bin_bx};
+        const int sizes[4] = {p.aln_x, p.aln_y, p.bin_y, p.bin_x};
+        for (int i = 0; i < 4; i++) {
+            make_fft(sizes[i], *fds[i]);
+            if ((st = upload(&p.d_tw_bin[i], twiddles(sizes[i]), B)) != ATR_OK) return st;
+            fds[i]->tw = p.d_tw_bin[i];
+        }
     }
     size_t sb = storage_bytes(p.storage);
     if ((st = dalloc(&p.d_sino, (size_t)p.vol_y * p.n * p.sino_stride * sb, B)) != ATR_OK) return st;
@@ -387,7 +441,8 @@ static void plan_free(Plan& p)
                     p.d_tw_z, p.d_tw_x, p.d_axis_cs, p.d_shifts, p.d_excl, p.d_aligned,
                     p.d_row_stats, p.d_tilt_stats, p.d_sino, p.d_diff, p.d_vol, p.d_sflag,
                     p.d_vflag, p.d_vstats, p.d_wbp_t, p.d_wbp_u, p.d_bp_args, p.d_fp_args,
-                    p.d_wbp_grid, p.d_wbp_tiles};
+                    p.d_wbp_grid, p.d_wbp_tiles, p.d_local, p.d_binned, p.d_bin_t, p.d_bin_u,
+                    p.d_tw_bin[0], p.d_tw_bin[1], p.d_tw_bin[2], p.d_tw_bin[3]};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& e : p.ev_pool) (void)hipEventDestroy(e);
@@ -447,16 +502,26 @@ static void timed(Plan& p, int kernel, hipStream_t s, F&& fn)
 }
 
 // ---- execution -----------------------------------------------------------
+// Pre-stage (alignment + fill statistics) and Fourier binning: what the
+// weighting / filter kernels read afterwards is stage_source(p, in).
+static void stage_prepare(Plan& p, const float* in, hipStream_t s)
+{
+    if (p.prestage || p.binned) {
+        timed(p, K_PRESTAGE, s, [&] {
+            if (p.prestage) {
+                launch_prestage(p, in, s);
+                launch_tilt_stats(p, s);
+            }
+            if (p.binned) launch_bin(p, in, s);
+        });
+    }
+}
+
 // SART before the iterations: pre-stage, weighting into the slice-major
 // sinogram, the volume state (zero, or the caller's SLICE volume).
 atr_status sart_prepare(Plan& p, const float* in, float* out, int accumulate, hipStream_t s)
 {
-    if (p.prestage) {
-        timed(p, K_PRESTAGE, s, [&] {
-            launch_prestage(p, in, s);
-            launch_tilt_stats(p, s);
-        });
-    }
+    stage_prepare(p, in, s);
     launch_zero_int(p.d_sflag, p.vol_y, s);
     launch_zero_int(p.d_vflag, p.vol_y, s);
     timed(p, K_WEIGHT, s, [&] { launch_weight_sart(p, in, s); });
@@ -491,12 +556,7 @@ static atr_status plan_execute(Plan& p, const float* in, float* out, int accumul
         }
         sart_finish(p, out, s);
     } else {
-        if (p.prestage) {
-            timed(p, K_PRESTAGE, s, [&] {
-                launch_prestage(p, in, s);
-                launch_tilt_stats(p, s);
-            });
-        }
+        stage_prepare(p, in, s);
         launch_zero_int(p.d_sflag, p.vol_y, s);
         timed(p, K_WEIGHT, s, [&] { launch_filter_wbp(p, in, s); });
         for (int y0 = 0, k = 0; y0 < p.vol_y; y0 += p.wbp_slab, k++) {
@@ -614,6 +674,17 @@ static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_
     return ATR_OK;
 }
 
+atr_status desc_volume(const atr_series_desc* d, int* vol_x, int* vol_y, int* vol_z)
+{
+    Plan p;
+    atr_status st = plan_sizes(d, p);
+    if (st != ATR_OK) return st;
+    *vol_x = p.vol_x;
+    *vol_y = p.vol_y;
+    *vol_z = p.vol_z;
+    return ATR_OK;
+}
+
 }  // namespace atr
 
 // ============================================================================
@@ -628,6 +699,7 @@ void atr_series_desc_init(atr_series_desc* d)
     if (!d) return;
     std::memset(d, 0, sizeof(*d));
     d->random_fill = 1;
+    d->bin = 1.0f;
     d->vol_z = 100;
     d->method = ATR_METHOD_SART;
     d->sart_iters = 20;   // -Sart 20 3 (CAtInput.cpp:69-70)

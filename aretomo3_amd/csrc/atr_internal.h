@@ -80,6 +80,9 @@ struct Plan {
     int in_x = 0, in_y = 0;   // input image size
     size_t in_tilt_stride = 0, in_row_stride = 0;
     int aln_x = 0, aln_y = 0; // aligned image (= input when no pre-stage)
+    bool binned = false;      // Fourier binning of the aligned images (-AtBin > 1)
+    int bin_x = 0, bin_y = 0; // binned image size
+    int n_patches = 0;        // local alignment patches per tilt (0 = global only)
     int proj = 0;         // effective projection width (ProjX/2*2)
     int vol_x = 0, vol_y = 0, vol_z = 0;
     int sino_stride = 0;  // elements per sinogram row on the device
@@ -124,6 +127,12 @@ struct Plan {
     float* d_shifts = nullptr;        // [N][2] global shifts
     unsigned char* d_excl = nullptr;  // [N] excluded tilts
     float* d_aligned = nullptr;       // [N][aln_y][aln_x] pre-stage output with sentinels
+    float* d_local = nullptr;         // [N][5][n_patches] patch centres, shifts, good flags
+    float* d_binned = nullptr;        // [N][bin_y][bin_x] Fourier-binned images
+    float2* d_bin_t = nullptr;        // [N][aln_y][bin_x/2+1] row spectra
+    float2* d_bin_u = nullptr;        // [N][bin_y][bin_x/2+1] cropped, column-inverted
+    float2* d_tw_bin[4] = {nullptr, nullptr, nullptr, nullptr};
+    FftDesc fft_bin_ax{}, fft_bin_ay{}, fft_bin_by{}, fft_bin_bx{};
     double* d_row_stats = nullptr;    // [N][aln_y][2] per-row (sum, count) of non-empty pixels
     double* d_tilt_stats = nullptr;   // [N][2] (sum, count) of non-empty aligned pixels
     void* d_sino = nullptr;           // [vol_y][N][sino_stride] weighted projections
@@ -200,6 +209,20 @@ struct FpItem {
     int item0, n_groups;       // items of one slice = tilt groups
 };
 
+// What the weighting / filter kernels read: the binned images, the
+// pre-stage output (with the random fill), or the caller's input.
+struct StageSource {
+    const float* src;
+    size_t tilt_stride, row_stride;
+    bool fill;
+};
+inline StageSource stage_source(const Plan& p, const float* in)
+{
+    if (p.binned) return {p.d_binned, (size_t)p.bin_x * p.bin_y, (size_t)p.bin_x, false};
+    if (p.prestage) return {p.d_aligned, (size_t)p.aln_x * p.aln_y, (size_t)p.aln_x, p.fill};
+    return {in, p.in_tilt_stride, p.in_row_stride, false};
+}
+
 // ---- SART stages shared by single-plan and batched execution (api.cpp) ----
 atr_status sart_prepare(Plan& p, const float* in, float* out, int accumulate, hipStream_t s);
 void sart_finish(Plan& p, float* out, hipStream_t s);
@@ -239,6 +262,10 @@ uint64_t splitmix64(uint64_t x);
 // ---- kernel launchers (kernels_*.hip) -----------------------------------
 void launch_prestage(const Plan& p, const float* in, hipStream_t s);
 void launch_tilt_stats(const Plan& p, hipStream_t s);
+void launch_bin(const Plan& p, const float* in, hipStream_t s);
+// Volume dimensions a plan for *d would produce (validates the descriptor).
+atr_status desc_volume(const atr_series_desc* d, int* vol_x, int* vol_y, int* vol_z);
+int bin_col_block(int ay);
 void launch_weight_sart(const Plan& p, const float* in, hipStream_t s);
 void launch_filter_wbp(const Plan& p, const float* in, hipStream_t s);
 void launch_zero(float* ptr, size_t n, hipStream_t s);

@@ -5,7 +5,9 @@
 // Stockham auto-sort formulation: each stage reads `src`, writes `dst` in
 // natural-order-preserving positions, so no bit reversal is needed and the
 // result of an s-stage transform sits in (s even ? a : b).  Radices 4, 2, 3
-// and 5 cover every size the plans accept (2^a 3^b 5^c, up to 8192).
+// and 5 have register butterflies; any other prime factor p is a direct
+// length-p DFT per output (p complex products each), so every size up to
+// 8192 is accepted and sizes with large prime factors cost O(n p).
 // Twiddles come from a table W[e] = (cos 2 pi e/n, sin 2 pi e/n) computed in
 // double on the host and rounded to float.  All arithmetic is FP32.
 #pragma once
@@ -103,6 +105,42 @@ __device__ __forceinline__ void fft_stage(const float2* __restrict__ src, float2
     }
 }
 
+// Stage of an arbitrary radix R (a prime above 5), one output per thread:
+// output q of butterfly (j, k = j mod ns) is sum_r src[j + r m] W^(r c),
+// c = k n/(ns R) + q m, which folds the inter-stage twiddle W^(r k step)
+// and the length-R DFT kernel W_R^(q r) = W^(q r m) into one table index.
+__device__ __forceinline__ void fft_stage_any(const float2* __restrict__ src, float2* __restrict__ dst,
+                                              int n, int batch, int ns, int R,
+                                              const float2* __restrict__ tw, float sign, int tid,
+                                              int nthreads)
+{
+    const int m = n / R;
+    const int step = n / (ns * R);
+    const int total = batch * n;
+    for (int idx = tid; idx < total; idx += nthreads) {
+        int bi = idx / n;
+        int pos = idx - bi * n;
+        int k = pos % ns;
+        int t = pos / ns;
+        int q = t % R;
+        int j = (t / R) * ns + k;
+        const float2* s = src + bi * n;
+        int c = k * step + q * m;
+        int e = 0;
+        float2 acc = make_float2(0.0f, 0.0f);
+        for (int r = 0; r < R; r++) {
+            float2 w = tw[e];
+            w.y *= sign;
+            float2 v = s[j + r * m];
+            acc.x += v.x * w.x - v.y * w.y;
+            acc.y += v.x * w.y + v.y * w.x;
+            e += c;
+            if (e >= n) e -= n;
+        }
+        dst[bi * n + pos] = acc;
+    }
+}
+
 // Transforms `batch` sequences of length fd.n stored contiguously in `a`
 // (b is scratch of the same size).  sign = -1 forward, +1 inverse
 // (unnormalised).  Returns the buffer holding the result.  Must be called by
@@ -118,7 +156,8 @@ __device__ __forceinline__ float2* lds_fft(float2* a, float2* b, const FftDesc& 
         if (r == 4) fft_stage<4>(src, dst, fd.n, batch, ns, fd.tw, sign, tid, nthreads);
         else if (r == 2) fft_stage<2>(src, dst, fd.n, batch, ns, fd.tw, sign, tid, nthreads);
         else if (r == 3) fft_stage<3>(src, dst, fd.n, batch, ns, fd.tw, sign, tid, nthreads);
-        else fft_stage<5>(src, dst, fd.n, batch, ns, fd.tw, sign, tid, nthreads);
+        else if (r == 5) fft_stage<5>(src, dst, fd.n, batch, ns, fd.tw, sign, tid, nthreads);
+        else fft_stage_any(src, dst, fd.n, batch, ns, r, fd.tw, sign, tid, nthreads);
         __syncthreads();
         ns *= r;
         float2* t = src;

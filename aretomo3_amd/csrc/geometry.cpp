@@ -296,7 +296,8 @@ bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, 
     return true;
 }
 
-// Factorisation into the radices fft_lds.h provides.
+// Factorisation into FFT stages: radix 4, 2, 3, 5 butterflies first, then
+// one direct-DFT stage per remaining prime factor.
 bool make_fft(int n, FftDesc& fd)
 {
     fd = FftDesc{};
@@ -307,6 +308,10 @@ bool make_fft(int n, FftDesc& fd)
     while (m % 2 == 0 && s < ATR_FFT_MAX_STAGES) { fd.radix[s++] = 2; m /= 2; }
     while (m % 3 == 0 && s < ATR_FFT_MAX_STAGES) { fd.radix[s++] = 3; m /= 3; }
     while (m % 5 == 0 && s < ATR_FFT_MAX_STAGES) { fd.radix[s++] = 5; m /= 5; }
+    // remaining prime factors: direct-DFT stages (fft_stage_any)
+    for (int q = 7; q * q <= m && s < ATR_FFT_MAX_STAGES; q += 2)
+        while (m % q == 0 && s < ATR_FFT_MAX_STAGES) { fd.radix[s++] = q; m /= q; }
+    if (m > 1 && s < ATR_FFT_MAX_STAGES) { fd.radix[s++] = m; m = 1; }
     fd.n_stages = s;
     return m == 1;
 }

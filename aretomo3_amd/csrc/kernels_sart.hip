@@ -19,21 +19,33 @@ namespace atr {
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // ------------------------------------------------------------------------
-// Pre-stage: Correct/GCorrPatchShift.cu:58-100 (global shift + tilt-axis
-// rotation, nearest neighbour, -1e30 outside the raw image), geometry in
-// FP64 with contraction off so that the rounding of every coordinate is the
-// rounding of the FP64 restatement.  One workgroup per (aligned row, tilt);
-// the row's (sum, count) of non-empty pixels is written for the fill mean.
+// Pre-stage: Correct/GCorrPatchShift.cu:58-100 (global + local shift,
+// tilt-axis rotation, nearest neighbour, -1e30 outside the raw image),
+// geometry in FP64 with contraction off so that the rounding of every
+// coordinate is the rounding of the FP64 restatement.  One workgroup per
+// (aligned row, tilt); the row's (sum, count) of non-empty pixels is written
+// for the fill mean.  The tilt's patch table (centres x, y, shifts x, y,
+// good flags; n_patches each) is staged in LDS.
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_prestage(const float* __restrict__ in, size_t tilt_stride,
                                                   size_t row_stride, int in_x, int in_y,
                                                   const double2* __restrict__ axis_cs,
                                                   const float2* __restrict__ shifts,
+                                                  const float* __restrict__ local, int n_patches,
                                                   float* __restrict__ out, int out_x, int out_y,
                                                   double2* __restrict__ row_stats)
 {
 #pragma clang fp contract(off)
+    extern __shared__ float s_patch[];
     const int y = blockIdx.x, t = blockIdx.y;
+    if (n_patches > 0) {
+        const float* lp = local + (size_t)t * 5 * n_patches;
+        for (int i = threadIdx.x; i < 5 * n_patches; i += blockDim.x) s_patch[i] = lp[i];
+        __syncthreads();
+    }
+    const float *pcx = s_patch, *pcy = s_patch + n_patches;
+    const float *psx = s_patch + 2 * n_patches, *psy = s_patch + 3 * n_patches;
+    const float* pgood = s_patch + 4 * n_patches;
     const double ca = axis_cs[t].x, sa = axis_cs[t].y;
     const double sx = (double)shifts[t].x + in_x * 0.5;
     const double sy = (double)shifts[t].y + in_y * 0.5;
@@ -46,6 +58,26 @@ __global__ __launch_bounds__(256) void k_prestage(const float* __restrict__ in, 
         double xc = x + 0.5 - out_x * 0.5;
         double xr = xc * ca - yc * sa;
         double yr = xc * sa + yc * ca;
+        if (n_patches > 0) {
+            // Correct/GCorrPatchShift.cu:18-48: Gaussian-weighted mean of the
+            // good patches' shifts (good >= 0.9); no weight -> no shift
+            double wx = 0.0, wy = 0.0, sw = 0.0;
+            int used = 0;
+            for (int q = 0; q < n_patches; q++) {
+                if (pgood[q] < 0.9f) continue;
+                double dx = (xr - (double)pcx[q]) / in_x;
+                double dy = (yr - (double)pcy[q]) / in_y;
+                double w = exp(-100.0 * (dx * dx + dy * dy));
+                wx += (double)psx[q] * w;
+                wy += (double)psy[q] * w;
+                sw += w;
+                used += 1;
+            }
+            if (used > 0 && sw > 0) {
+                xr += wx / sw;
+                yr += wy / sw;
+            }
+        }
         xr += sx;
         yr += sy;
         int ix = (int)(xr + 0.5);
@@ -752,9 +784,10 @@ void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s)
 void launch_prestage(const Plan& p, const float* in, hipStream_t s)
 {
     dim3 grid(p.aln_y, p.n);
-    hipLaunchKernelGGL(k_prestage, grid, dim3(256), 0, s, in, p.in_tilt_stride, p.in_row_stride,
-                       p.in_x, p.in_y, (const double2*)p.d_axis_cs, (const float2*)p.d_shifts,
-                       p.d_aligned, p.aln_x, p.aln_y, (double2*)p.d_row_stats);
+    hipLaunchKernelGGL(k_prestage, grid, dim3(256), sizeof(float) * 5 * (size_t)p.n_patches, s, in,
+                       p.in_tilt_stride, p.in_row_stride, p.in_x, p.in_y,
+                       (const double2*)p.d_axis_cs, (const float2*)p.d_shifts, p.d_local,
+                       p.n_patches, p.d_aligned, p.aln_x, p.aln_y, (double2*)p.d_row_stats);
 }
 
 void launch_tilt_stats(const Plan& p, hipStream_t s)
@@ -766,11 +799,11 @@ void launch_tilt_stats(const Plan& p, hipStream_t s)
 template <typename S>
 static void launch_weight_t(const Plan& p, const float* in, hipStream_t s)
 {
-    const float* src = p.prestage ? p.d_aligned : in;
-    size_t ts = p.prestage ? (size_t)p.aln_x * p.aln_y : p.in_tilt_stride;
-    size_t rs = p.prestage ? (size_t)p.aln_x : p.in_row_stride;
+    const StageSource src0 = stage_source(p, in);
+    const float* src = src0.src;
+    const size_t ts = src0.tilt_stride, rs = src0.row_stride;
     dim3 grid((p.sino_stride + 255) / 256, p.n, p.vol_y);
-    hipLaunchKernelGGL(k_weight<S>, grid, dim3(256), 0, s, src, ts, rs, p.fill ? 1 : 0,
+    hipLaunchKernelGGL(k_weight<S>, grid, dim3(256), 0, s, src, ts, rs, src0.fill ? 1 : 0,
                        p.d_fill_aln, (const double2*)p.d_tilt_stats, p.d_cs, p.n, p.proj,
                        p.vol_z, (S*)p.d_sino, p.sino_stride, p.d_sflag);
 }

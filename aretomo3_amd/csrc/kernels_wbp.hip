@@ -11,6 +11,8 @@
 //                    Recon/CDoWbpRecon.cpp:107-123)
 // Scratch: d_wbp_grid [slab][2VolZ][2VolX], d_wbp_t [slab][2VolZ][VolX/2+1],
 // d_wbp_u [slab][VolZ][VolX/2+1].
+#include <algorithm>
+
 #include "atr_internal.h"
 #include "storage.h"
 
@@ -31,6 +33,128 @@ __device__ __forceinline__ float filled_value_w(const float* __restrict__ img, i
         return w;
     }
     return mean;
+}
+
+// ------------------------------------------------------------------------
+// Fourier binning of the aligned stack (Correct/CBinStack.cpp:54-106,
+// -AtBin > 1): per image, R2C over ax x ay normalised by 1/(ax ay), crop to
+// bx x by (MaUtil/GFourierResize2D.cu:7-39), unnormalised C2R.
+//   k_bin_rowfft  two image rows packed as one complex row, FFT of length
+//                 ax, the columns kx < bx/2+1 of both half spectra kept
+//   k_bin_colfft  per block of columns: FFT of length ay, normalisation,
+//                 crop to by rows, inverse FFT of length by
+//   k_bin_rowc2r  two binned rows: packed C2R of length bx
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bin_rowfft(const float* __restrict__ in, size_t tilt_stride,
+                                                    size_t row_stride, int fill,
+                                                    const int* __restrict__ table,
+                                                    const double2* __restrict__ tilt_stats,
+                                                    int ax, int ay, int kxb, FftDesc fd,
+                                                    float2* __restrict__ tbuf)
+{
+    extern __shared__ float2 lds[];
+    float2* a = lds;
+    float2* b = lds + ax;
+    const int t = blockIdx.y;
+    const int y0 = 2 * blockIdx.x, y1 = y0 + 1;
+    const bool has1 = y1 < ay;
+    const float* img = in + (size_t)t * tilt_stride;
+    float mean = 0.0f;
+    if (fill) {
+        double2 ts = tilt_stats[t];
+        mean = ts.y > 0 ? (float)(ts.x / ts.y) : 0.0f;
+    }
+    for (int x = threadIdx.x; x < ax; x += blockDim.x) {
+        float va, vb = 0.0f;
+        if (fill) {
+            va = filled_value_w(img, (int)(y0 * row_stride + x), table, mean);
+            if (has1) vb = filled_value_w(img, (int)(y1 * row_stride + x), table, mean);
+        } else {
+            va = img[(size_t)y0 * row_stride + x];
+            if (has1) vb = img[(size_t)y1 * row_stride + x];
+        }
+        a[x] = make_float2(va, vb);
+    }
+    __syncthreads();
+    float2* r = lds_fft(a, b, fd, 1, -1.0f, threadIdx.x, blockDim.x);
+    float2* ta = tbuf + ((size_t)t * ay + y0) * kxb;
+    float2* tb = tbuf + ((size_t)t * ay + y1) * kxb;
+    for (int k = threadIdx.x; k < kxb; k += blockDim.x) {
+        float2 z1 = r[k];
+        float2 z2 = r[(ax - k) % ax];
+        ta[k] = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+        if (has1) tb[k] = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_colfft(const float2* __restrict__ tbuf,
+                                                    float2* __restrict__ ubuf, int ax, int ay,
+                                                    int by, int kxb, int kb, FftDesc fy,
+                                                    FftDesc fby)
+{
+    extern __shared__ float2 lds[];
+    float2* a = lds;
+    float2* b = lds + kb * ay;
+    const int k0 = blockIdx.x * kb, t = blockIdx.y;
+    const int nk = min(kb, kxb - k0);
+    const float2* src = tbuf + (size_t)t * ay * kxb;
+    for (int i = threadIdx.x; i < kb * ay; i += blockDim.x) {
+        int col = i % kb, z = i / kb;
+        a[col * ay + z] = col < nk ? src[(size_t)z * kxb + k0 + col] : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    float2* r = lds_fft(a, b, fy, kb, -1.0f, threadIdx.x, blockDim.x);
+    float2* o = (r == a) ? b : a;
+    const float norm = (float)(1.0 / ((double)ax * (double)ay));
+    for (int i = threadIdx.x; i < kb * by; i += blockDim.x) {
+        int col = i / by, j = i % by;
+        int srcz = j <= by / 2 ? j : j - by + ay;
+        float2 v = r[col * ay + srcz];
+        o[col * by + j] = make_float2(v.x * norm, v.y * norm);
+    }
+    __syncthreads();
+    float2* q = lds_fft(o, r, fby, kb, 1.0f, threadIdx.x, blockDim.x);
+    float2* dst = ubuf + (size_t)t * by * kxb;
+    for (int i = threadIdx.x; i < kb * by; i += blockDim.x) {
+        int col = i % kb, z = i / kb;
+        if (col < nk) dst[(size_t)z * kxb + k0 + col] = q[col * by + z];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_rowc2r(const float2* __restrict__ ubuf, int bx, int by,
+                                                    FftDesc fx, float* __restrict__ out)
+{
+    extern __shared__ float2 lds[];
+    float2* a = lds;
+    float2* b = lds + bx;
+    const int kxb = bx / 2 + 1, half = bx / 2;
+    const int t = blockIdx.y, ya = 2 * blockIdx.x, yb = ya + 1;
+    const bool has_b = yb < by;
+    const float2* ha = ubuf + ((size_t)t * by + ya) * kxb;
+    const float2* hb = ubuf + ((size_t)t * by + (has_b ? yb : ya)) * kxb;
+    for (int k = threadIdx.x; k < bx; k += blockDim.x) {
+        float2 va, vb;
+        if (k == 0 || k == half) {
+            va = make_float2(ha[k].x, 0.0f);
+            vb = make_float2(hb[k].x, 0.0f);
+        } else if (k < half) {
+            va = ha[k];
+            vb = hb[k];
+        } else {
+            float2 u = ha[bx - k], w = hb[bx - k];
+            va = make_float2(u.x, -u.y);
+            vb = make_float2(w.x, -w.y);
+        }
+        if (!has_b) vb = make_float2(0.0f, 0.0f);
+        a[k] = make_float2(va.x - vb.y, va.y + vb.x);
+    }
+    __syncthreads();
+    float2* r = lds_fft(a, b, fx, 1, 1.0f, threadIdx.x, blockDim.x);
+    float* oa = out + ((size_t)t * by + ya) * bx;
+    for (int x = threadIdx.x; x < bx; x += blockDim.x) {
+        oa[x] = r[x].x;
+        if (has_b) oa[bx + x] = r[x].y;
+    }
 }
 
 // One workgroup per (tilt, pair of rows): the two weighted rows are packed as
@@ -226,12 +350,12 @@ __global__ __launch_bounds__(256) void k_wbp_rowc2r(const float2* __restrict__ u
 template <typename S>
 static void launch_filter_t(const Plan& p, const float* in, hipStream_t s)
 {
-    const float* src = p.prestage ? p.d_aligned : in;
-    size_t ts = p.prestage ? (size_t)p.aln_x * p.aln_y : p.in_tilt_stride;
-    size_t rs = p.prestage ? (size_t)p.aln_x : p.in_row_stride;
+    const StageSource src0 = stage_source(p, in);
+    const float* src = src0.src;
+    const size_t ts = src0.tilt_stride, rs = src0.row_stride;
     dim3 grid((p.vol_y + 1) / 2, p.n);
     size_t shm = 2 * (size_t)p.proj * sizeof(float2);
-    hipLaunchKernelGGL(k_filter<S>, grid, dim3(256), shm, s, src, ts, rs, p.fill ? 1 : 0,
+    hipLaunchKernelGGL(k_filter<S>, grid, dim3(256), shm, s, src, ts, rs, src0.fill ? 1 : 0,
                        p.d_fill_aln, (const double2*)p.d_tilt_stats, p.d_cs, p.n, p.vol_y, p.proj,
                        p.vol_z, p.fft_proj, p.d_filter, (S*)p.d_sino, p.sino_stride, p.d_sflag);
 }
@@ -276,6 +400,31 @@ void launch_wbp_rowc2r(const Plan& p, int y0, int ny, float* out, hipStream_t s)
     size_t shm = 2 * (size_t)p.vol_x * sizeof(float2);
     hipLaunchKernelGGL(k_wbp_rowc2r, grid, dim3(256), shm, s, p.d_wbp_u, p.vol_x, p.vol_y,
                        p.vol_z, y0, p.fft_x, p.layout, out);
+}
+
+// Fourier binning of the stage input into p.d_binned.
+void launch_bin(const Plan& p, const float* in, hipStream_t s)
+{
+    const float* src = p.prestage ? p.d_aligned : in;
+    size_t ts = p.prestage ? (size_t)p.aln_x * p.aln_y : p.in_tilt_stride;
+    size_t rs = p.prestage ? (size_t)p.aln_x : p.in_row_stride;
+    const int ax = p.aln_x, ay = p.aln_y, bx = p.bin_x, by = p.bin_y;
+    const int kxb = bx / 2 + 1;
+    hipLaunchKernelGGL(k_bin_rowfft, dim3((ay + 1) / 2, p.n), dim3(256), 2 * (size_t)ax * sizeof(float2),
+                       s, src, ts, rs, p.fill ? 1 : 0, p.d_fill_aln, (const double2*)p.d_tilt_stats,
+                       ax, ay, kxb, p.fft_bin_ax, p.d_bin_t);
+    const int kb = bin_col_block(ay);
+    hipLaunchKernelGGL(k_bin_colfft, dim3((kxb + kb - 1) / kb, p.n), dim3(256),
+                       2 * (size_t)kb * ay * sizeof(float2), s, p.d_bin_t, p.d_bin_u, ax, ay, by,
+                       kxb, kb, p.fft_bin_ay, p.fft_bin_by);
+    hipLaunchKernelGGL(k_bin_rowc2r, dim3((by + 1) / 2, p.n), dim3(256), 2 * (size_t)bx * sizeof(float2),
+                       s, p.d_bin_u, bx, by, p.fft_bin_bx, p.d_binned);
+}
+
+int bin_col_block(int ay)
+{
+    int kb = 4096 / ay;
+    return std::max(1, std::min(kb, 16));
 }
 
 }  // namespace atr

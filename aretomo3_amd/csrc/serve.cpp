@@ -33,7 +33,7 @@ constexpr int kRing = 4;   // rounds whose item arrays may be in flight at once
 struct Request {
     uint64_t id = 0;
     atr_series_desc desc{};
-    std::vector<float> tilts, axes, shifts;
+    std::vector<float> tilts, axes, shifts, local;
     std::vector<unsigned char> excl;
     int with_wbp = 0;
     const float* proj_host = nullptr;
@@ -401,6 +401,10 @@ atr_status atr_server_submit(atr_server* s, const atr_series_desc* desc, int wit
         r->excl.assign(desc->exclude, desc->exclude + n);
         r->desc.exclude = r->excl.data();
     }
+    if (desc->n_patches > 0 && desc->local_params) {
+        r->local.assign(desc->local_params, desc->local_params + (size_t)n * 5 * desc->n_patches);
+        r->desc.local_params = r->local.data();
+    }
     size_t bytes_sart = 0, bytes_wbp = 0;
     atr_status st = atr_estimate_device_bytes(&r->desc, &bytes_sart);
     if (st != ATR_OK) return st;
@@ -411,9 +415,9 @@ atr_status atr_server_submit(atr_server* s, const atr_series_desc* desc, int wit
     const size_t row = desc->row_stride ? desc->row_stride : (size_t)desc->proj_x;
     const size_t tstride = desc->tilt_stride ? desc->tilt_stride : row * desc->proj_y;
     r->in_bytes = sizeof(float) * (tstride * (n - 1) + row * (desc->proj_y - 1) + desc->proj_x);
-    int ax = desc->proj_x, ay = desc->proj_y;
-    if (desc->tilt_axis_deg) aligned_size(desc->proj_x, desc->proj_y, desc->tilt_axis_deg[0], &ax, &ay);
-    r->out_bytes = sizeof(float) * (size_t)(ax / 2 * 2) * ay * desc->vol_z;
+    int vx = 0, vy = 0, vz = 0;
+    if ((st = desc_volume(&r->desc, &vx, &vy, &vz)) != ATR_OK) return st;
+    r->out_bytes = sizeof(float) * (size_t)vx * vy * vz;
     r->wbp_bytes = with_wbp ? r->out_bytes : 0;
     r->need = bytes_sart + bytes_wbp + r->in_bytes + r->out_bytes + r->wbp_bytes;
     r->with_wbp = with_wbp;

@@ -82,8 +82,13 @@ def series_desc(proj_x, proj_y, tilts, vol_z, method=L.METHOD_SART, *, tilt_axis
                 shifts=None, random_fill=True, exclude=None, sart_iters=20, sart_subsets=1,
                 sart_start_tilt=0, sart_n_tilts=0, relax=-1.0,
                 subset_order=L.SUBSETS_CONTIGUOUS, positivity=True, storage=L.STORAGE_F32,
-                layout=L.LAYOUT_XZY, fill_seed=0, tilt_stride=0, row_stride=0):
-    """An atr_series_desc and the arrays it points to (keep both alive)."""
+                layout=L.LAYOUT_XZY, fill_seed=0, tilt_stride=0, row_stride=0,
+                local_params=None, bin=1.0):
+    """An atr_series_desc and the arrays it points to (keep both alive).
+
+    local_params: per-tilt patch tables [n][5][n_patches] (centres x, centres
+    y, shifts x, shifts y, good flags) of CLocalAlignParam; bin: -AtBin
+    Fourier binning of the aligned images (1 = none)."""
     t = np.ascontiguousarray(tilts, np.float32)
     n = len(t)
     keep = [t]
@@ -113,6 +118,12 @@ def series_desc(proj_x, proj_y, tilts, vol_z, method=L.METHOD_SART, *, tilt_axis
     d.subset_order, d.positivity = int(subset_order), 1 if positivity else 0
     d.storage, d.out_layout = int(storage), int(layout)
     d.fill_seed = int(fill_seed)
+    if local_params is not None:
+        lp = np.ascontiguousarray(local_params, np.float32).reshape(n, 5, -1)
+        keep.append(lp)
+        d.n_patches = int(lp.shape[2])
+        d.local_params = lp.ctypes.data
+    d.bin = float(bin)
     return d, keep
 
 

@@ -105,6 +105,17 @@ typedef struct atr_series_desc {
     int storage;                /* atr_storage */
     int out_layout;             /* atr_layout */
     uint64_t fill_seed;         /* seeds the empty-fill index tables */
+    /* local alignment (CLocalAlignParam, Correct/GCorrPatchShift.cu:18-48):
+     * per tilt n_patches centres x, then centres y, shifts x, shifts y and
+     * good flags, i.e. [n_tilts][5][n_patches]; 0 / NULL = global only.
+     * Requires the pre-stage (tilt_axis_deg != NULL). */
+    int n_patches;
+    const float* local_params;
+    /* -AtBin (AreTomo/CAreTomoMain.cpp:737-753, Correct/CBinStack.cpp:54-106):
+     * Fourier binning of the aligned images to ((int)(size/bin))/2*2 before
+     * the reconstruction; 1 = none.  The random fill is then forced on
+     * (Correct/CCorrTomoStack.cpp:107).  vol_z is the binned thickness. */
+    float bin;
 } atr_series_desc;
 
 typedef struct atr_plan atr_plan;

@@ -132,10 +132,32 @@ def test_rejects_non_finite_angles_and_small_strides():
 
 def test_wbp_size_limits():
     d, keep = make_desc(method=L.METHOD_WBP)
-    d.proj_x = 4 * 7 * 11     # 2*VolX has factor 7, 11: not a 2/3/5 FFT size
-    assert estimate(d)[0] == L.ATR_ERR_UNSUPPORTED_SIZE
+    d.proj_x = 4 * 7 * 11     # prime factors 7, 11: direct-DFT FFT stages
+    assert estimate(d)[0] == L.ATR_OK
     d.proj_x = 600
     assert estimate(d)[0] == L.ATR_OK
+    d.proj_x = 4100           # ProjX above 4096
+    assert estimate(d)[0] == L.ATR_ERR_UNSUPPORTED_SIZE
+
+
+def test_local_and_bin_validation():
+    d, keep = make_desc()
+    lp = np.zeros((d.n_tilts, 5, 4), np.float32)
+    d.n_patches, d.local_params = 4, lp.ctypes.data
+    assert estimate(d)[0] == L.ATR_ERR_INVALID_ARG          # no pre-stage
+    axes = np.full(d.n_tilts, 85.0, np.float32)
+    d.tilt_axis_deg = axes.ctypes.data
+    assert estimate(d)[0] == L.ATR_OK
+    lp[1, 2, 3] = np.nan
+    assert estimate(d)[0] == L.ATR_ERR_INVALID_ARG
+    lp[1, 2, 3] = 0.0
+    d.local_params = None
+    assert estimate(d)[0] == L.ATR_ERR_INVALID_ARG
+    d.n_patches = 0
+    for b, want in ((1.0, L.ATR_OK), (2.0, L.ATR_OK), (1.5, L.ATR_OK), (0.5, L.ATR_ERR_INVALID_ARG),
+                    (float("nan"), L.ATR_ERR_INVALID_ARG), (1e6, L.ATR_ERR_INVALID_ARG)):
+        d.bin = b
+        assert estimate(d)[0] == want, b
 
 
 def test_null_arguments_fail_cleanly():
