@@ -5,13 +5,14 @@ loop, with the stack-correction pre-stage that feeds them.
 
 All computation runs in the HIP library ``libatr_recon.so`` (C ABI in
 ``include/aretomo_recon.h``); this package is the host-side mirror of the
-reference operators (``recon``) and the synthetic-data generator
-(``phantom``).
+reference operators (``recon``), the on-disk formats and reconstruct-from-files
+driver (``files``, C ABI in ``include/aretomo_files.h``) and the synthetic-data
+generator (``phantom``).
 """
-from . import _lib
+from . import _lib, files
 from ._lib import AtrError
 from .recon import (Batch, DeviceArray, DoSartRecon, DoWbpRecon, PinnedArray, Plan, Server,
                     TomoSart, TomoWbp, device_count, series_desc, synchronize)
 
-__all__ = ["AtrError", "Batch", "DeviceArray", "DoSartRecon", "DoWbpRecon", "PinnedArray", "Plan",
+__all__ = ["AtrError", "files", "Batch", "DeviceArray", "DoSartRecon", "DoWbpRecon", "PinnedArray", "Plan",
            "Server", "TomoSart", "TomoWbp", "device_count", "series_desc", "synchronize", "_lib"]

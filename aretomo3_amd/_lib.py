@@ -22,7 +22,7 @@ LAYOUT_XZY, LAYOUT_XYZ, LAYOUT_XYZ_NOFLIP, LAYOUT_SLICE = 0, 1, 2, 3
 # kernel ids of atr_plan_kernel_stats
 K_BP, K_FP, K_WEIGHT, K_WBP_BP, K_PRESTAGE, K_OUTPUT, K_WBP_COL, K_WBP_ROW = range(8)
 
-# every symbol include/aretomo_recon.h declares
+# every symbol include/aretomo_recon.h and include/aretomo_files.h declare
 EXPORTS = [
     "atr_series_desc_init", "atr_plan_create", "atr_plan_get_volume_size",
     "atr_plan_device_bytes", "atr_estimate_device_bytes", "atr_execute", "atr_execute_host",
@@ -37,6 +37,9 @@ EXPORTS = [
     "atr_server_config_init", "atr_server_create", "atr_server_submit", "atr_server_wait",
     "atr_server_stats", "atr_server_destroy", "atr_host_alloc", "atr_host_free",
     "atr_plan_autotune", "atr_plan_set_tile_shape",
+    "atr_aligned_size", "atr_mrc_read_info", "atr_mrc_read_sections", "atr_mrc_write", "atr_volume_suffix",
+    "atr_aln_read", "atr_aln_get_info", "atr_aln_get_global", "atr_aln_get_local",
+    "atr_aln_get_dark", "atr_aln_destroy", "atr_aln_write", "atr_tilt_file_read",
 ]
 BATCH_GRAPH = 1
 
@@ -83,6 +86,21 @@ class RequestInfo(ctypes.Structure):
     _fields_ = [("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
                 ("done_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
                 ("status", ctypes.c_int)]
+
+
+class MrcInfo(ctypes.Structure):
+    """Mirror of atr_mrc_info."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("mode", ctypes.c_int), ("pixel_size", ctypes.c_float), ("nsymbt", ctypes.c_int),
+                ("swapped", ctypes.c_int)]
+
+
+class AlnInfo(ctypes.Structure):
+    """Mirror of atr_aln_info."""
+    _fields_ = [("raw_size", ctypes.c_int * 3), ("n_tilts", ctypes.c_int),
+                ("n_patches", ctypes.c_int), ("n_dark", ctypes.c_int),
+                ("alpha_offset", ctypes.c_float), ("beta_offset", ctypes.c_float),
+                ("thickness", ctypes.c_int)]
 
 
 class AtrError(RuntimeError):
@@ -155,6 +173,20 @@ def lib():
         "atr_host_free": (i, [vp]),
         "atr_plan_autotune": (i, [vp, vp, pi, pi, pi]),
         "atr_plan_set_tile_shape": (i, [vp, i, i, i]),
+        "atr_aligned_size": (i, [i, i, f, pi, pi]),
+        "atr_mrc_read_info": (i, [ctypes.c_char_p, ctypes.POINTER(MrcInfo)]),
+        "atr_mrc_read_sections": (i, [ctypes.c_char_p, i, i, vp]),
+        "atr_mrc_write": (i, [ctypes.c_char_p, vp, i, i, i, f, i]),
+        "atr_volume_suffix": (ctypes.c_char_p, [i]),
+        "atr_aln_read": (i, [ctypes.c_char_p, ctypes.POINTER(vp)]),
+        "atr_aln_get_info": (i, [vp, ctypes.POINTER(AlnInfo)]),
+        "atr_aln_get_global": (i, [vp, vp, vp, vp, vp]),
+        "atr_aln_get_local": (i, [vp, vp]),
+        "atr_aln_get_dark": (i, [vp, vp, vp, vp]),
+        "atr_aln_destroy": (None, [vp]),
+        "atr_aln_write": (i, [ctypes.c_char_p, ctypes.POINTER(AlnInfo), vp, vp, vp, vp, vp, vp,
+                              vp, vp]),
+        "atr_tilt_file_read": (i, [ctypes.c_char_p, i, vp, vp, vp, pi]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

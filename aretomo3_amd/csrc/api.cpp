@@ -738,6 +738,15 @@ atr_status atr_estimate_device_bytes(const atr_series_desc* desc, size_t* bytes)
     return ATR_OK;
 }
 
+atr_status atr_aligned_size(int raw_x, int raw_y, float tilt_axis_deg, int* aln_x, int* aln_y)
+{
+    if (!aln_x || !aln_y) return fail(ATR_ERR_INVALID_ARG, "atr_aligned_size: NULL argument");
+    if (raw_x < 1 || raw_y < 1 || !std::isfinite(tilt_axis_deg))
+        return fail(ATR_ERR_INVALID_ARG, "atr_aligned_size: bad image size or axis");
+    aligned_size(raw_x, raw_y, tilt_axis_deg, aln_x, aln_y);
+    return ATR_OK;
+}
+
 atr_status atr_plan_get_volume_size(const atr_plan* plan, int* vx, int* vy, int* vz)
 {
     if (!plan) return fail(ATR_ERR_INVALID_ARG, "plan is NULL");

@@ -135,6 +135,9 @@ atr_status atr_plan_get_volume_size(const atr_plan* plan, int* vol_x, int* vol_y
 size_t atr_plan_device_bytes(const atr_plan* plan);
 /* Device bytes a plan for *desc would allocate (serving admission). */
 atr_status atr_estimate_device_bytes(const atr_series_desc* desc, size_t* bytes);
+/* Aligned image size of the pre-stage for a raw image and the tilt axis of
+ * frame 0 (CCorrectUtil::CalcAlignedSize, Correct/CCorrectUtil.cpp:11-21). */
+atr_status atr_aligned_size(int raw_x, int raw_y, float tilt_axis_deg, int* aln_x, int* aln_y);
 
 /* Runs the reconstruction on `stream`.  proj_dev: device input laid out as
  * described in the plan's descriptor; vol_dev: device output in out_layout.

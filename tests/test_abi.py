@@ -12,13 +12,15 @@ from aretomo3_amd import _lib as L
 from aretomo3_amd import phantom
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "aretomo_recon.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("aretomo_recon.h", "aretomo_files.h")]
 
 
 def header_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(atr_[a-z0-9_]+)\s*\(", text)))
+    found = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        found.update(re.findall(r"\b(atr_[a-z0-9_]+)\s*\(", text))
+    return sorted(found)
 
 
 def test_header_matches_binding_and_library():
