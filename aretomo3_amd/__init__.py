@@ -9,10 +9,10 @@ reference operators (``recon``), the on-disk formats and reconstruct-from-files
 driver (``files``, C ABI in ``include/aretomo_files.h``) and the synthetic-data
 generator (``phantom``).
 """
-from . import _lib, files
+from . import _lib, files, projalign
 from ._lib import AtrError
 from .recon import (Batch, DeviceArray, DoSartRecon, DoWbpRecon, PinnedArray, Plan, Server,
                     TomoSart, TomoWbp, device_count, series_desc, synchronize)
 
-__all__ = ["AtrError", "files", "Batch", "DeviceArray", "DoSartRecon", "DoWbpRecon", "PinnedArray", "Plan",
+__all__ = ["AtrError", "files", "projalign", "Batch", "DeviceArray", "DoSartRecon", "DoWbpRecon", "PinnedArray", "Plan",
            "Server", "TomoSart", "TomoWbp", "device_count", "series_desc", "synchronize", "_lib"]

@@ -40,6 +40,7 @@ EXPORTS = [
     "atr_aligned_size", "atr_mrc_read_info", "atr_mrc_read_sections", "atr_mrc_write", "atr_volume_suffix",
     "atr_aln_read", "atr_aln_get_info", "atr_aln_get_global", "atr_aln_get_local",
     "atr_aln_get_dark", "atr_aln_destroy", "atr_aln_write", "atr_tilt_file_read",
+    "atr_reproj_create", "atr_reproj_execute", "atr_reproj_find_range", "atr_reproj_destroy",
 ]
 BATCH_GRAPH = 1
 
@@ -187,6 +188,10 @@ def lib():
         "atr_aln_write": (i, [ctypes.c_char_p, ctypes.POINTER(AlnInfo), vp, vp, vp, vp, vp, vp,
                               vp, vp]),
         "atr_tilt_file_read": (i, [ctypes.c_char_p, i, vp, vp, vp, pi]),
+        "atr_reproj_create": (i, [i, i, i, i, ctypes.POINTER(vp)]),
+        "atr_reproj_execute": (i, [vp, vp, vp, vp, i, vp, vp]),
+        "atr_reproj_find_range": (i, [vp, vp, i, i, pi, pi]),
+        "atr_reproj_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -209,6 +209,21 @@ struct FpItem {
     int item0, n_groups;       // items of one slice = tilt groups
 };
 
+// Re-projector launch arguments (passed by value): tilts [first, first +
+// n_range) back-projected with host FP64 cos / sin / cos(a - t) tables, then
+// a ray of `ray` samples at (ray_cos, ray_sin) per output pixel, empty below
+// min_count non-empty samples.
+constexpr int REPROJ_MAX_TILTS = 10;
+struct ReprojArgs {
+    int proj_x, proj_y, vol_x, vol_z;
+    int first, n_range, ray;
+    float min_count;
+    double cos_t[REPROJ_MAX_TILTS], sin_t[REPROJ_MAX_TILTS], weight[REPROJ_MAX_TILTS];
+    double ray_cos, ray_sin;
+};
+void launch_reproj(const ReprojArgs& a, const float* projs, float* vol_slab, int slab_rows,
+                   float* out, hipStream_t s);
+
 // What the weighting / filter kernels read: the binned images, the
 // pre-stage output (with the random fill), or the caller's input.
 struct StageSource {

@@ -278,6 +278,29 @@ atr_status atr_event_destroy(void* ev);
 atr_status atr_event_record(void* ev, void* stream);
 atr_status atr_event_elapsed_ms(void* start, void* stop, float* ms);
 
+/* ---- ProjAlign re-projector (AreTomo/ProjAlign/CCalcReproj.cpp,
+ * GReproj.cu) --------------------------------------------------------------
+ * Predicts projection proj_idx from its lower-tilt neighbours: tilts within
+ * 20.5 deg whose cos stretch cos t / cos a is at most 1.2 (skip[i] != 0
+ * excludes tilt i; at most 10; else the neighbour toward 0 deg) are back-
+ * projected, nearest sample weighted by cos t * cos(a - t), onto a
+ * 2 proj_x x vol_z slice per image row and forward-projected at angle a
+ * (-1e30 where a ray meets fewer than 80 % non-empty voxels). */
+typedef struct atr_reproj atr_reproj;
+/* CCalcReproj::Setup(piTomoStkSize, iVolZ) (CCalcReproj.cpp:36-45). */
+atr_status atr_reproj_create(int proj_x, int proj_y, int n_tilts, int vol_z, atr_reproj** reproj);
+/* CCalcReproj::DoIt(ppfProjs, pfTiltAngles, pbSkipProjs, iProjIdx, pfReproj)
+ * (CCalcReproj.cpp:47-65): proj_dev [n_tilts][proj_y][proj_x] and
+ * reproj_dev [proj_y][proj_x] on the device; tilts / skip (may be NULL) on
+ * the host.  Stream-ordered. */
+atr_status atr_reproj_execute(atr_reproj* reproj, const float* proj_dev, const float* tilts,
+                              const unsigned char* skip, int proj_idx, float* reproj_dev,
+                              void* stream);
+/* The tilt range of CCalcReproj::mFindProjRange (CCalcReproj.cpp:73-124). */
+atr_status atr_reproj_find_range(const float* tilts, const unsigned char* skip, int n_tilts,
+                                 int proj_idx, int* first, int* last);
+void atr_reproj_destroy(atr_reproj* reproj);
+
 #ifdef __cplusplus
 }
 #endif

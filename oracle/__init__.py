@@ -56,6 +56,8 @@ def lib():
                                          _c_int, _c_int]),
             "orc_wbp_crop": (None, [_dp, _c_int, _c_int, _dp]),
             "orc_fourier_crop": (None, [_dp, _c_int, _c_int, _c_int, _c_int, _dp]),
+            "orc_reproj_range": (None, [_fp, _c_vp, _c_int, _c_int, _ip]),
+            "orc_reproj": (None, [_fp, _fp, _c_int, _c_int, _c_int, _c_int, _c_int, _ip, _dp]),
             "orc_bin_size": (None, [_c_int, _c_int, _c_f, ctypes.POINTER(_c_int),
                                     ctypes.POINTER(_c_int)]),
             "orc_bin_stack": (None, [_dp, _c_int, _c_int, _c_int, _c_int, _c_int, _dp]),
@@ -303,6 +305,27 @@ def recon_slices(aligned, tilts, method, vol_z, slices, start_tilt=0, n_sart_til
     lib().orc_recon_slices(a, n, px, py, _f32(tilts), mp, method, vol_z, start_tilt,
                            n_sart_tilts, iters, n_subsets, relax, order, positivity, seed,
                            sl, len(sl), out)
+    return out
+
+
+def reproj_range(tilts, proj_idx, skip=None):
+    """CCalcReproj::mFindProjRange -> (first, last) tilt index."""
+    t = _f32(tilts)
+    m, mp = _mask(skip)
+    r = np.zeros(2, np.int32)
+    lib().orc_reproj_range(t, mp, len(t), int(proj_idx), r)
+    return int(r[0]), int(r[1])
+
+
+def reproj(projs, tilts, vol_z, proj_idx, skip=None, rng=None):
+    """CCalcReproj::DoIt: projs [n][proj_y][proj_x] float32 -> [proj_y][proj_x]."""
+    p = _f32(projs)
+    n, py, px = p.shape
+    if rng is None:
+        rng = reproj_range(tilts, proj_idx, skip)
+    r = np.asarray(rng, np.int32)
+    out = np.empty((py, px), np.float64)
+    lib().orc_reproj(p, _f32(tilts), n, px, py, int(vol_z), int(proj_idx), r, out)
     return out
 
 

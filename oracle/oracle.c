@@ -688,6 +688,112 @@ void orc_recon_slices(const double* aligned, int n_tilts, int proj_x,
 
 /* DataUtil/CTiltSeries.cpp:286-305: XZY frames (frame y holds vol_z rows)
  * -> XYZ frames; new frame index = flip ? (vol_z-1-row) : row. */
+/* ---------------------------------------------------------------------- */
+/* ProjAlign re-projector: AreTomo/ProjAlign/CCalcReproj.cpp:45-153 and    */
+/* GReproj.cu:22-203.  Angles use the reference's float constant 0.01745f  */
+/* (GReproj.cu:6, CCalcReproj.cpp:10); geometry in double.                 */
+/* ---------------------------------------------------------------------- */
+
+/* CCalcReproj::mFindProjRange (CCalcReproj.cpp:73-124): the first and last
+ * non-skipped tilt within 20.5 deg of the projection angle whose cos stretch
+ * cos(t)/cos(a) is at most 1.2; at most 10 tilts; when none qualifies, the
+ * neighbour one step toward 0 deg.  range[0..1] inclusive. */
+void orc_reproj_range(const float* tilts, const unsigned char* skip, int n,
+                      int proj_idx, int* range)
+{
+    const float d2r = 0.01745f;
+    float a = tilts[proj_idx];
+    double cos_a = cos((double)(a * d2r));
+    int start = -1, end = -1;
+    for (int i = 0; i < n && start < 0; i++) {
+        if (skip && skip[i]) continue;
+        if (fabs((double)(a - tilts[i])) > 20.5) continue;
+        if ((float)(cos((double)(tilts[i] * d2r)) / cos_a) > 1.20f) continue;
+        start = i;
+    }
+    for (int i = start < 0 ? n : start; i < n; i++) {
+        if (skip && skip[i]) continue;
+        if (fabs((double)(a - tilts[i])) > 20.5) continue;
+        if ((float)(cos((double)(tilts[i] * d2r)) / cos_a) > 1.20f) continue;
+        end = i;
+    }
+    if (start < 0 || end < 0) {
+        int sign = a > 0 ? 1 : -1;
+        start = end = proj_idx - sign;
+    }
+    if (end - start > 9) end = start + 9;
+    range[0] = start;
+    range[1] = end;
+}
+
+/* GReproj::DoIt per row y: mGBackProj (GReproj.cu:22-59) of tilts
+ * [range0, range1] onto a 2 proj_x x vol_z slice -- nearest sample
+ * (truncation) at u = x cos + z sin + proj_x/2, weighted by cos t *
+ * cos(a - t), normalised by the weight sum (-1e30 below 0.001) -- then
+ * mGForProj (:63-110) at angle a along rays of length
+ * L = (int)(vol_z / cos a + 0.5f): mean of the non-empty samples, -1e30 when
+ * fewer than 0.8 L.  projs [n][proj_y][proj_x], out [proj_y][proj_x]. */
+void orc_reproj(const float* projs, const float* tilts, int n_tilts, int proj_x,
+                int proj_y, int vol_z, int proj_idx, const int* range, double* out)
+{
+    const float d2r = 0.01745f;
+    const int vol_x = 2 * proj_x;
+    const float a = tilts[proj_idx];
+    const int r0 = range[0], r1 = range[1];
+    const int nr = r1 - r0 + 1;
+    double* cs = (double*)malloc(sizeof(double) * 3 * (size_t)(nr > 0 ? nr : 1));
+    for (int k = 0; k < nr; k++) {
+        double v = (double)tilts[r0 + k] * (double)d2r;
+        cs[3 * k] = cos(v);
+        cs[3 * k + 1] = sin(v);
+        cs[3 * k + 2] = cos(((double)a - (double)tilts[r0 + k]) * (double)d2r);
+    }
+    const double rad = 4.0 * atan(1.0) / 180.0;
+    const float fc = (float)cos(rad * a), fs = (float)sin(rad * a);
+    const int ray = (int)((float)vol_z / fc + 0.5f);
+    const float thr = 0.8f * (float)ray;
+    (void)n_tilts;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int y = 0; y < proj_y; y++) {
+        float* vol = (float*)malloc(sizeof(float) * (size_t)vol_x * vol_z);
+        for (int z = 0; z < vol_z; z++) {
+            double fz = z + 0.5 - vol_z * 0.5;
+            for (int x = 0; x < vol_x; x++) {
+                double fx = x + 0.5 - vol_x * 0.5;
+                double sum = 0.0, cnt = 0.0;
+                for (int k = 0; k < nr; k++) {
+                    double u = fx * cs[3 * k] + fz * cs[3 * k + 1] + proj_x * 0.5;
+                    if (u < 0 || u > proj_x - 1) continue;
+                    float v = projs[((size_t)(r0 + k) * proj_y + y) * proj_x + (int)u];
+                    if (v < (float)ORC_EMPTY) continue;
+                    sum += v * cs[3 * k] * cs[3 * k + 2];
+                    cnt += cs[3 * k + 2];
+                }
+                vol[(size_t)z * vol_x + x] = cnt < 0.001 ? (float)ORC_SENT : (float)(sum / cnt);
+            }
+        }
+        const double tz = vol_z * 0.5 - ray * 0.5 * (double)fc;
+        for (int x = 0; x < proj_x; x++) {
+            double xp = x + 0.5 - proj_x * 0.5;
+            double tx = vol_x * 0.5 + ray * 0.5 * (double)fs + xp / (double)fc;
+            double sum = 0.0;
+            int cnt = 0;
+            for (int i = 0; i < ray; i++) {
+                double px = tx - i * (double)fs;
+                double pz = tz + i * (double)fc;
+                if (px < 0 || px >= vol_x || pz < 0 || pz >= vol_z) continue;
+                float v = vol[(size_t)(int)pz * vol_x + (int)px];
+                if (v <= (float)ORC_EMPTY) continue;
+                sum += v;
+                cnt += 1;
+            }
+            out[(size_t)y * proj_x + x] = ((float)cnt < thr) ? ORC_SENT : sum / cnt;
+        }
+        free(vol);
+    }
+    free(cs);
+}
+
 void orc_flip_vol(const double* vol_xzy, int vol_x, int vol_z, int vol_y,
                   int flip, double* vol_xyz)
 {

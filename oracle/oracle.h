@@ -87,6 +87,10 @@ void orc_recon_slices(const double* aligned, int n_tilts, int proj_x,
                       int n_subsets, double relax, int order, int positivity,
                       uint64_t fill_seed, const int* slices, int n_slices,
                       double* out);
+void orc_reproj_range(const float* tilts, const unsigned char* skip, int n,
+                      int proj_idx, int* range);
+void orc_reproj(const float* projs, const float* tilts, int n_tilts, int proj_x,
+                int proj_y, int vol_z, int proj_idx, const int* range, double* out);
 void orc_flip_vol(const double* vol_xzy, int vol_x, int vol_z, int vol_y,
                   int flip, double* vol_xyz);
 int  orc_max_threads(void);

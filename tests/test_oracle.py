@@ -614,3 +614,73 @@ def test_bin_stack_sizes_and_dc():
     out = oracle.bin_stack(img, 2.0)[0]
     xb = np.arange(18)
     np.testing.assert_allclose(out, np.tile(np.cos(2 * np.pi * 3 * xb / 18), (12, 1)), atol=1e-12)
+
+
+# ---- ProjAlign re-projector (CCalcReproj / GReproj) ------------------------
+def test_reproj_range_cases():
+    t = np.arange(-60, 61, 3, dtype=np.float32)
+    idx = {float(a): i for i, a in enumerate(t)}
+    # |a - t| <= 20.5 with the angles at and above a skipped: 12..27 deg
+    assert oracle.reproj_range(t, idx[30.0], (np.abs(t) >= 30).astype(np.uint8)) == (idx[12.0], idx[27.0])
+    # cos stretch <= 1.2 at 60 deg: t >= 53.13 deg
+    assert oracle.reproj_range(t, idx[60.0], (t >= 60).astype(np.uint8)) == (idx[54.0], idx[57.0])
+    # at most 10 tilts: 1-degree steps, 40 deg: 24..39 capped to 24..33
+    t1 = np.arange(-60, 61, 1, dtype=np.float32)
+    assert oracle.reproj_range(t1, 100, (t1 >= 40).astype(np.uint8)) == (84, 93)
+    # nothing qualifies: the neighbour one step toward 0 deg
+    t5 = np.array([-60, -30, 0, 30, 60], np.float32)
+    assert oracle.reproj_range(t5, 4, np.array([0, 0, 0, 0, 1], np.uint8)) == (3, 3)
+    assert oracle.reproj_range(t5, 0, np.array([1, 0, 0, 0, 0], np.uint8)) == (1, 1)
+
+
+def test_reproj_zero_tilt_is_identity():
+    # one tilt at 0 deg re-projected at 0 deg: the BP voxel column x + px/2
+    # holds proj[x] for every z, the FP ray reads it vol_z times; the last
+    # column falls outside u <= px - 1 and is empty
+    rng = np.random.default_rng(4)
+    p = rng.uniform(1, 2, (1, 5, 32)).astype(np.float32)
+    r = oracle.reproj(p, np.array([0.0], np.float32), 12, 0, rng=(0, 0))
+    np.testing.assert_array_equal(r[:, :-1], p[0, :, :-1].astype(np.float64))
+    assert np.all(r[:, -1] == -1e30)
+
+
+def test_reproj_constant_and_aligned_rays():
+    # one tilt t re-projected at t: a constant c gives c cos t (the weight
+    # cos(a - t) = 1 cancels); a quadratic profile comes back within one
+    # pixel of itself, which a ray crossing the back-projection would not
+    for ang in (25.0, -40.0):
+        tl = np.array([ang], np.float32)
+        c = np.full((1, 4, 64), 3.0, np.float32)
+        r = oracle.reproj(c, tl, 16, 0, rng=(0, 0))
+        ok = r > -1e29
+        assert ok[:, 20:44].all()
+        want = np.float32(3.0 * np.cos(np.float64(np.float32(ang * np.float32(0.01745)))))
+        np.testing.assert_allclose(r[ok], want, rtol=1e-7)
+        u = np.arange(64, dtype=np.float64)
+        q = np.broadcast_to(((u - 32) ** 2).astype(np.float32), (1, 4, 64)).copy()
+        rq = oracle.reproj(q, tl, 16, 0, rng=(0, 0))[0] / float(want / 3.0)
+        for x in range(22, 42):
+            lo = min((x - 33) ** 2, (x - 31) ** 2, (x - 32) ** 2)
+            hi = max((x - 33) ** 2, (x - 31) ** 2)
+            assert lo - 1e-3 <= rq[x] <= hi + 1e-3, (ang, x, rq[x])
+
+
+def test_reproj_linear_and_sentinels():
+    rng = np.random.default_rng(5)
+    t = np.arange(-30, 31, 6, dtype=np.float32)
+    P = rng.uniform(0.5, 1.5, (len(t), 3, 40)).astype(np.float32)
+    Q = rng.uniform(0.5, 1.5, (len(t), 3, 40)).astype(np.float32)
+    skip = (t >= 18).astype(np.uint8)
+    k = int(np.where(t == 18)[0][0])
+    a = oracle.reproj(P, t, 10, k, skip)
+    b = oracle.reproj(Q, t, 10, k, skip)
+    ab = oracle.reproj((2 * P + 3 * Q).astype(np.float32), t, 10, k, skip)
+    ok = a > -1e29
+    assert ok.sum() > 60 and np.array_equal(ok, ab > -1e29)
+    np.testing.assert_allclose(ab[ok], 2 * a[ok] + 3 * b[ok], rtol=2e-6)
+    # a band empty in every projection of the range is empty in the output
+    Ps = P.copy()
+    Ps[:, :, 10:14] = -1e30
+    s = oracle.reproj(Ps, t, 10, k, skip)
+    assert np.all(s[:, 11:13] == -1e30)
+    assert np.all(s[:, 30:32] > -1e29)
