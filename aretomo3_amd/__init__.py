@@ -11,8 +11,8 @@ generator (``phantom``).
 """
 from . import _lib, files, projalign
 from ._lib import AtrError
-from .recon import (Batch, DeviceArray, DoSartRecon, DoWbpRecon, PinnedArray, Plan, Server,
+from .recon import (Batch, CalcVolThick, DeviceArray, DoSartRecon, DoWbpRecon, PinnedArray, Plan, Server,
                     TomoSart, TomoWbp, device_count, series_desc, synchronize)
 
-__all__ = ["AtrError", "files", "projalign", "Batch", "DeviceArray", "DoSartRecon", "DoWbpRecon", "PinnedArray", "Plan",
+__all__ = ["AtrError", "files", "projalign", "Batch", "CalcVolThick", "DeviceArray", "DoSartRecon", "DoWbpRecon", "PinnedArray", "Plan",
            "Server", "TomoSart", "TomoWbp", "device_count", "series_desc", "synchronize", "_lib"]

@@ -41,6 +41,7 @@ EXPORTS = [
     "atr_aln_read", "atr_aln_get_info", "atr_aln_get_global", "atr_aln_get_local",
     "atr_aln_get_dark", "atr_aln_destroy", "atr_aln_write", "atr_tilt_file_read",
     "atr_reproj_create", "atr_reproj_execute", "atr_reproj_find_range", "atr_reproj_destroy",
+    "atr_calc_vol_thick", "atr_thickness_edges",
 ]
 BATCH_GRAPH = 1
 
@@ -102,6 +103,13 @@ class AlnInfo(ctypes.Structure):
                 ("n_patches", ctypes.c_int), ("n_dark", ctypes.c_int),
                 ("alpha_offset", ctypes.c_float), ("beta_offset", ctypes.c_float),
                 ("thickness", ctypes.c_int)]
+
+
+class Thickness(ctypes.Structure):
+    """Mirror of atr_thickness."""
+    _fields_ = [("low_edge", ctypes.c_int), ("high_edge", ctypes.c_int), ("thickness", ctypes.c_int),
+                ("offset_z", ctypes.c_int), ("binning", ctypes.c_float),
+                ("pixel_size", ctypes.c_float), ("vol_z", ctypes.c_int), ("n_profile", ctypes.c_int)]
 
 
 class AtrError(RuntimeError):
@@ -192,6 +200,8 @@ def lib():
         "atr_reproj_execute": (i, [vp, vp, vp, vp, i, vp, vp]),
         "atr_reproj_find_range": (i, [vp, vp, i, i, pi, pi]),
         "atr_reproj_destroy": (None, [vp]),
+        "atr_calc_vol_thick": (i, [ctypes.POINTER(SeriesDesc), f, vp, ctypes.POINTER(Thickness), vp]),
+        "atr_thickness_edges": (i, [vp, i, f, ctypes.POINTER(Thickness), vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -431,3 +431,50 @@ class DoSartRecon:
             return plan.execute_host(ts)
         finally:
             plan.close()
+
+
+# --------------------------------------------------------------------------
+# sample thickness (CCalcVolThick)
+# --------------------------------------------------------------------------
+def thickness_edges(std_profile, binning):
+    """CCalcVolThick::mSmooth + mDetectEdges on a std profile ->
+    (atr_thickness fields as a dict, smoothed profile)."""
+    p = np.ascontiguousarray(std_profile, np.float32)
+    out = L.Thickness()
+    sm = np.empty_like(p)
+    check(lib().atr_thickness_edges(p.ctypes.data, len(p), float(binning), ctypes.byref(out),
+                                    sm.ctypes.data))
+    return {k: getattr(out, k) for k, _ in L.Thickness._fields_}, sm
+
+
+class CalcVolThick:
+    """CCalcVolThick: DoIt on a raw tilt series (CCalcVolThick.cpp:47-90),
+    then GetThickness / GetLowEdge / GetHighEdge (:28-45)."""
+
+    def __init__(self):
+        self.result = None
+        self.profile = None
+
+    def do_it(self, raw, tilts, pixel_size, tilt_axis=None, shifts=None, exclude=None,
+              fill_seed=0):
+        raw = np.ascontiguousarray(raw, np.float32)
+        n, py, px = raw.shape
+        d, keep = series_desc(px, py, tilts, 100, L.METHOD_WBP, tilt_axis=tilt_axis,
+                              shifts=shifts, exclude=exclude, fill_seed=fill_seed)
+        out = L.Thickness()
+        prof = np.zeros(max(px, py) + 8, np.float32)
+        check(lib().atr_calc_vol_thick(ctypes.byref(d), float(pixel_size), raw.ctypes.data,
+                                       ctypes.byref(out), prof.ctypes.data))
+        self.result = {k: getattr(out, k) for k, _ in L.Thickness._fields_}
+        self.profile = prof[:out.n_profile].copy()
+        return self.result
+
+    def get_thickness(self, angstrom=False):
+        t = float(self.result["high_edge"] - self.result["low_edge"])
+        return t * self.result["pixel_size"] if angstrom else t
+
+    def get_low_edge(self, angstrom=False):
+        return self.result["low_edge"] * (self.result["pixel_size"] if angstrom else 1.0)
+
+    def get_high_edge(self, angstrom=False):
+        return self.result["high_edge"] * (self.result["pixel_size"] if angstrom else 1.0)

@@ -301,6 +301,34 @@ atr_status atr_reproj_find_range(const float* tilts, const unsigned char* skip, 
                                  int proj_idx, int* first, int* last);
 void atr_reproj_destroy(atr_reproj* reproj);
 
+/* ---- Sample thickness (Recon/CCalcVolThick.cpp) ---------------------------
+ * The other in-repo caller of the reconstruction: a WBP of the series
+ * Fourier-binned by max(20 / pixel_size, 20), flipped to xyz, the standard
+ * deviation of the central 40 % tile of each z section, an 11-point moving
+ * average and the edge search of CCalcVolThick::mDetectEdges.  Edges are in
+ * aligned (unbinned) pixels; thickness is CAlignParam::m_iThickness and
+ * offset_z CAlignParam::m_iOffsetZ. */
+typedef struct atr_thickness {
+    int low_edge, high_edge;  /* m_aiSampleEdges */
+    int thickness;            /* high_edge - low_edge */
+    int offset_z;             /* sample centre - volume centre */
+    float binning;            /* m_fBinning */
+    float pixel_size;         /* m_fPixSize: Angstrom per aligned pixel */
+    int vol_z;                /* sections of the binned volume */
+    int n_profile;            /* vol_z - 1 profile values */
+} atr_thickness;
+/* CCalcVolThick::DoIt (CCalcVolThick.cpp:47-90): desc describes the raw
+ * series (size, strides, tilts, tilt axis, shifts, exclude, fill seed); its
+ * method, bin, vol_z, layout and storage are replaced.  proj_host: host
+ * input.  profile (may be NULL) receives the unsmoothed std profile,
+ * out->n_profile values. */
+atr_status atr_calc_vol_thick(const atr_series_desc* desc, float pixel_size, const float* proj_host,
+                              atr_thickness* out, float* profile);
+/* mSmooth + mDetectEdges (CCalcVolThick.cpp:93-110,185-255) on a std profile
+ * of n >= 10 sections; smoothed (may be NULL) receives the smoothed profile. */
+atr_status atr_thickness_edges(const float* std_profile, int n, float binning, atr_thickness* out,
+                               float* smoothed);
+
 #ifdef __cplusplus
 }
 #endif

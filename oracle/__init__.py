@@ -58,6 +58,9 @@ def lib():
             "orc_fourier_crop": (None, [_dp, _c_int, _c_int, _c_int, _c_int, _dp]),
             "orc_reproj_range": (None, [_fp, _c_vp, _c_int, _c_int, _ip]),
             "orc_reproj": (None, [_fp, _fp, _c_int, _c_int, _c_int, _c_int, _c_int, _ip, _dp]),
+            "orc_thick_profile": (None, [_fp, _c_int, _c_int, _c_int, _fp]),
+            "orc_thick_smooth": (None, [_fp, _c_int]),
+            "orc_thick_edges": (None, [_fp, _c_int, _c_f, _ip]),
             "orc_bin_size": (None, [_c_int, _c_int, _c_f, ctypes.POINTER(_c_int),
                                     ctypes.POINTER(_c_int)]),
             "orc_bin_stack": (None, [_dp, _c_int, _c_int, _c_int, _c_int, _c_int, _dp]),
@@ -327,6 +330,45 @@ def reproj(projs, tilts, vol_z, proj_idx, skip=None, rng=None):
     out = np.empty((py, px), np.float64)
     lib().orc_reproj(p, _f32(tilts), n, px, py, int(vol_z), int(proj_idx), r, out)
     return out
+
+
+def thick_profile(vol_xyz):
+    """CCalcVolThick::mMeasureStd over z = 0 .. nz-2 of an xyz volume [nz][ny][nx]."""
+    v = _f32(vol_xyz)
+    nz, ny, nx = v.shape
+    out = np.zeros(max(nz - 1, 0), np.float32)
+    lib().orc_thick_profile(v, nx, ny, nz, out)
+    return out
+
+
+def thick_smooth(profile):
+    p = _f32(profile).copy()
+    lib().orc_thick_smooth(p, len(p))
+    return p
+
+
+def thick_edges(smoothed, binning):
+    """CCalcVolThick::mDetectEdges -> (low edge, high edge, thickness, offset z)."""
+    out = np.zeros(4, np.int32)
+    lib().orc_thick_edges(_f32(smoothed), len(smoothed), float(binning), out)
+    return tuple(int(v) for v in out)
+
+
+def calc_vol_thick(raw, tilts, axes, shifts, pixel_size, seed=0):
+    """CCalcVolThick::DoIt on the CPU: pre-stage with fill, Fourier binning by
+    max(20 / pixel, 20), WBP of thickness (int)(0.8 binned x)/2*2, z-flipped xyz
+    volume, std profile, smoothing, edges."""
+    binning = max(np.float32(20.0) / np.float32(pixel_size), np.float32(20.0))
+    aligned = prestage(raw, axes, shifts, 1, seed) if axes is not None else _f64(raw)
+    binned = bin_stack(aligned, float(binning))
+    n, by, bx = binned.shape
+    vol_z = int(bx * 0.8) // 2 * 2
+    vol = recon_slices(binned, tilts, 0, vol_z, list(range(by)), seed=seed)
+    xyz = flip_vol(vol, 1).astype(np.float32)
+    prof = thick_profile(xyz)
+    sm = thick_smooth(prof)
+    return dict(edges=thick_edges(sm, binning), profile=prof, smoothed=sm, vol_z=vol_z,
+                binning=float(binning), volume=xyz)
 
 
 def flip_vol(vol_xzy, flip=1):

@@ -794,6 +794,109 @@ void orc_reproj(const float* projs, const float* tilts, int n_tilts, int proj_x,
     free(cs);
 }
 
+/* ---------------------------------------------------------------------- */
+/* Sample thickness: Recon/CCalcVolThick.cpp:47-285.                      */
+/* ---------------------------------------------------------------------- */
+
+/* CCalcVolThick::mSetup + mMeasureStd (CCalcVolThick.cpp:116-170): the
+ * standard deviation of the central tile ((int)(0.4 n)/2*2 per side) of
+ * sections z = 0 .. nz-2 of an xyz volume, samples below -1e20 left out;
+ * moments in double, sqrt taken in float as the reference does. */
+void orc_thick_profile(const float* vol, int nx, int ny, int nz, float* out)
+{
+    int tx = (int)(nx * 0.4f) / 2 * 2, ty = (int)(ny * 0.4f) / 2 * 2;
+    int x0 = (nx - tx) / 2, y0 = (ny - ty) / 2;
+    for (int z = 0; z + 1 < nz; z++) {
+        double s1 = 0.0, s2 = 0.0;
+        int cnt = 0;
+        for (int j = 0; j < ty; j++) {
+            const float* row = vol + ((size_t)z * ny + y0 + j) * nx + x0;
+            for (int i = 0; i < tx; i++) {
+                float v = row[i];
+                if (v < (float)-1e20) continue;
+                s1 += v;
+                s2 += (double)(v * v);
+                cnt += 1;
+            }
+        }
+        if (cnt == 0) {
+            out[z] = 0.0f;
+            continue;
+        }
+        s1 = s1 / cnt;
+        s2 = s2 / cnt - s1 * s1;
+        out[z] = s2 <= 0 ? 0.0f : sqrtf((float)s2);
+    }
+}
+
+/* CCalcVolThick::mSmooth (:93-110): 11-point moving average with the ends
+ * clamped, double sum then (float) sum / 11. */
+void orc_thick_smooth(float* ccs, int n)
+{
+    const int win = 11;
+    float* buf = (float*)malloc(sizeof(float) * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        double sum = 0.0;
+        for (int j = 0; j < win; j++) {
+            int k = i - win / 2 + j;
+            if (k < 0) k = 0;
+            else if (k >= n) k = n - 1;
+            sum += ccs[k];
+        }
+        buf[i] = (float)sum / win;
+    }
+    memcpy(ccs, buf, sizeof(float) * (size_t)n);
+    free(buf);
+}
+
+/* CCalcVolThick::mDetectEdges (:185-255) on the smoothed profile: mean of
+ * [n/10, n - n/10) in float, the first crossing of the mean from each end
+ * averaged with the lowest point outside it, scaled by the binning (int *=
+ * float).  out: low edge, high edge, thickness, z offset of the sample
+ * centre from the volume centre.  A side with no crossing keeps its search
+ * bound (the reference leaves the edge uninitialised). */
+void orc_thick_edges(const float* ccs, int n, float binning, int* out)
+{
+    int half = n / 2;
+    int bot = (int)(n * 0.1), top = n - bot;
+    float mean = 0.0f;  /* the reference's std of the profile is unused */
+    for (int i = bot; i < top; i++) mean += ccs[i];
+    mean = mean / (top - bot);
+    int e0 = bot, e1 = top;
+    for (int i = bot; i < half; i++) {
+        if (ccs[i] < mean) continue;
+        e0 = i;
+        break;
+    }
+    float mn = mean;
+    int imin = e0;
+    for (int i = bot; i < e0; i++) {
+        if (ccs[i] >= mn) continue;
+        mn = ccs[i];
+        imin = i;
+    }
+    e0 = (e0 + imin) / 2;
+    for (int i = top; i > half; i--) {
+        if (ccs[i] < mean) continue;
+        e1 = i;
+        break;
+    }
+    mn = mean;
+    imin = e1;
+    for (int i = top; i > e1; i--) {
+        if (ccs[i] >= mn) continue;
+        mn = ccs[i];
+        imin = i;
+    }
+    e1 = (e1 + imin) / 2;
+    e0 = (int)(e0 * binning);
+    e1 = (int)(e1 * binning);
+    out[0] = e0;
+    out[1] = e1;
+    out[2] = e1 - e0;
+    out[3] = (e0 + e1) / 2 - (int)(half * binning);
+}
+
 void orc_flip_vol(const double* vol_xzy, int vol_x, int vol_z, int vol_y,
                   int flip, double* vol_xyz)
 {

@@ -91,6 +91,9 @@ void orc_reproj_range(const float* tilts, const unsigned char* skip, int n,
                       int proj_idx, int* range);
 void orc_reproj(const float* projs, const float* tilts, int n_tilts, int proj_x,
                 int proj_y, int vol_z, int proj_idx, const int* range, double* out);
+void orc_thick_profile(const float* vol, int nx, int ny, int nz, float* out);
+void orc_thick_smooth(float* ccs, int n);
+void orc_thick_edges(const float* ccs, int n, float binning, int* out);
 void orc_flip_vol(const double* vol_xzy, int vol_x, int vol_z, int vol_y,
                   int flip, double* vol_xyz);
 int  orc_max_threads(void);

@@ -684,3 +684,51 @@ def test_reproj_linear_and_sentinels():
     s = oracle.reproj(Ps, t, 10, k, skip)
     assert np.all(s[:, 11:13] == -1e30)
     assert np.all(s[:, 30:32] > -1e29)
+
+
+# ---- sample thickness (CCalcVolThick) ---------------------------------------
+def test_thick_profile_is_the_tile_std():
+    rng = np.random.default_rng(8)
+    nz, ny, nx = 7, 30, 41
+    vol = rng.normal(0, 1, (nz, ny, nx)).astype(np.float32) * np.arange(1, nz + 1)[:, None, None]
+    vol = vol.astype(np.float32)
+    tx, ty = int(nx * np.float32(0.4)) // 2 * 2, int(ny * np.float32(0.4)) // 2 * 2
+    x0, y0 = (nx - tx) // 2, (ny - ty) // 2
+    vol[2, y0 + 1, x0 + 2] = -1e30                       # an empty voxel is left out
+    vol[:, :y0, :] = 1e6                                 # outside the tile: ignored
+    prof = oracle.thick_profile(vol)
+    assert prof.shape == (nz - 1,)
+    for z in range(nz - 1):
+        tile = vol[z, y0:y0 + ty, x0:x0 + tx].astype(np.float64)
+        tile = tile[tile > -1e20]
+        np.testing.assert_allclose(prof[z], tile.std(), rtol=1e-6)
+    assert np.all(oracle.thick_profile(np.full((4, 10, 10), 3.0, np.float32)) == 0)
+
+
+def test_thick_smooth_closed_forms():
+    imp = np.zeros(40, np.float32)
+    imp[20] = 11.0
+    s = oracle.thick_smooth(imp)
+    np.testing.assert_array_equal(s[15:26], np.ones(11, np.float32))
+    assert s[14] == 0 and s[26] == 0
+    end = np.zeros(20, np.float32)
+    end[0] = 1.0
+    s = oracle.thick_smooth(end)
+    assert s[0] == np.float32(6.0) / 11 and s[5] == np.float32(1.0) / 11 and s[6] == 0
+    np.testing.assert_array_equal(oracle.thick_smooth(np.full(15, 2.5, np.float32)), 2.5)
+
+
+def test_thick_edges_box_profile():
+    # box of 1 on [30, 70) in 0: mean over [10, 90) = 0.5; the low edge is the
+    # midpoint of the first point >= mean (30) and the first minimum (10), the
+    # high edge of 69 and 90
+    p = np.zeros(100, np.float32)
+    p[30:70] = 1.0
+    assert oracle.thick_edges(p, 1.0) == (20, 79, 59, 49 - 50)
+    # binning scales the edges (int *= float) and the volume centre
+    assert oracle.thick_edges(p, 20.0) == (400, 1580, 1180, 990 - 1000)
+    # a sloped floor: the lowest point outside the crossing is used
+    q = p.copy()
+    q[10:30] = np.linspace(0.4, 0.1, 20, dtype=np.float32)
+    e0 = oracle.thick_edges(q, 1.0)[0]
+    assert e0 == (30 + 29) // 2
