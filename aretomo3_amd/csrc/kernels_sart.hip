@@ -460,9 +460,10 @@ __device__ __forceinline__ float fp_sample(const char* __restrict__ band, float4
 }
 
 // Samples [0, count) of one (pair, band): geometry entries G[0..count),
-// sample indices k0 + m.  Groups of four run without tests; the final
-// partial group clamps and drops the samples past the window.  MASKED also
-// tests each sample against the ray's FP64 interval rr = (first, count).
+// sample indices k0 + m.  Groups of four (one 64-byte scalar load of
+// geometry), then the remainder one sample at a time.  MASKED clamps the
+// coordinate and tests each sample against the ray's FP64 interval
+// rr = (first, count).
 template <bool MASKED, bool SENT>
 __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
                                        cptr<float4> G, int count, int k0,
@@ -490,23 +491,20 @@ __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
             }
         }
     }
-    if (m < count) {
-        const Geo4 g4 = ldc((cptr<Geo4>)(G + m));
-        float v[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            // entries past the window may belong to another band: read row 0
-            const float4 g = m + q < count ? g4.g[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            const float xs = __builtin_amdgcn_fmed3f(tf1 - g4.g[q].z, 1.0f, xmax1);
-            v[q] = fp_sample(band, g, xs, rowb);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            bool in = m + q < count;
-            if (MASKED) in = in & ((unsigned)(k0 + m + q - rr.x) < (unsigned)rr.y);
-            if (SENT) in = in & !(v[q] < kEmpty);
-            acc += in ? v[q] : 0.0f;
+    // the last 1-3 samples one at a time: no lane computes a sample past
+    // the window
+    for (; m < count; m++) {
+        const float4 g = ldc(G + m);
+        float xs = tf1 - g.z;
+        if (MASKED) xs = __builtin_amdgcn_fmed3f(xs, 1.0f, xmax1);
+        const float v = fp_sample(band, g, xs, rowb);
+        if (MASKED) {
+            bool in = (unsigned)(k0 + m - rr.x) < (unsigned)rr.y;
+            if (SENT) in = in & !(v < kEmpty);
+            acc += in ? v : 0.0f;
             if (SENT) cnt += in ? 1 : 0;
+        } else {
+            acc += v;
         }
     }
 }
