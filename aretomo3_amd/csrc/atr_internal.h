@@ -34,7 +34,7 @@ constexpr int FP_WAVE = 64;
 constexpr int FP_MAXP = 16;
 constexpr int FP_LDS_BUDGET = 150 * 1024;
 constexpr int FP_BZ_MAX = 17;                // cell rows per band at most
-constexpr int FP_STAGE_ROWS = 6;             // rows a thread loads before forming pairs
+constexpr int FP_STAGE_ROWS = 9;              // rows a thread loads before forming pairs
 
 enum Kernel {
     K_BP = 0,

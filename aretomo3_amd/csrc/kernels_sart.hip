@@ -557,7 +557,12 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
         }
     }
     const char* bandb = (const char*)band;
-    for (int b = 0; b < nb; b++) {
+    // Work items start at different bands, so that the band reloads of the
+    // workgroups on the chip are spread out in time instead of all hitting
+    // HBM at once.  (A ray's samples are summed band by band in this order.)
+    const int rot = (int)(((unsigned)y * 5u + (unsigned)gx) % (unsigned)nb);
+    for (int bi = 0; bi < nb; bi++) {
+        const int b = bi + rot < nb ? bi + rot : bi + rot - nb;
         const int zb = b * bz;
         const int rows = min(bz, vol_z - 1 - zb) + 1;  // volume rows zb .. zb+rows-1
         // Staging: a thread owns whole columns of interpolation pairs

@@ -160,10 +160,11 @@ atr_status atr_execute_host(atr_plan* plan, const float* proj_host, float* vol_h
  * times the candidate tile shapes of a SART plan on its own geometry -- back-
  * projection z rows per thread {16, 8} x tilts per LDS stage {16, 8}, then
  * forward-projection band rows -- and keeps the fastest.  Choices are cached
- * per shape within the process.  Every voxel and every ray sums its terms
- * in the same order whatever the shape; only the FP32 rounding of the back-
- * projection coordinate (taken relative to the tile's window) can differ, at
- * the 1e-7 level.  Outputs may be NULL.  A tuned plan with a non-default back-projection
+ * per shape within the process.  Every voxel sums its tilts in the same
+ * order whatever the shape; the FP32 rounding of the back-projection
+ * coordinate (taken relative to the tile's window) and the order in which a
+ * forward ray adds up its z-bands depend on the shape, at the 1e-7 level.
+ * Outputs may be NULL.  A tuned plan with a non-default back-projection
  * shape cannot join a batch or be served. */
 atr_status atr_plan_autotune(atr_plan* plan, void* stream, int* bp_tz, int* bp_chunk,
                              int* fp_band_rows);
