@@ -367,7 +367,9 @@ static atr_status plan_build(Plan& p)
         if ((st = dalloc((void**)&p.d_vflag, sizeof(int) * p.vol_y, B)) != ATR_OK) return st;
         if ((st = dalloc((void**)&p.d_vstats, sizeof(double) * 2 * p.vol_y, B)) != ATR_OK) return st;
         p.device_bytes = B;
-        if ((st = sart_tables(p, p.bp_tz, p.bp_chunk, fp_band_rows(p.vol_x))) != ATR_OK) return st;
+        if ((st = sart_tables(p, p.bp_tz, p.bp_chunk,
+                              fp_balanced_rows(fp_band_rows(p.vol_x), p.vol_z))) != ATR_OK)
+            return st;
         B = p.device_bytes;
     } else {
         build_wbp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.wbp_lohi);
@@ -639,7 +641,8 @@ static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_
         ATR_HIP(hipMemsetAsync(p.d_vol, 0, sizeof(float) * p.vol_x * p.vol_y * p.vol_z, s));
         ATR_HIP(hipMemsetAsync(p.d_sflag, 0, sizeof(int) * p.vol_y, s));
         ATR_HIP(hipMemsetAsync(p.d_vflag, 0, sizeof(int) * p.vol_y, s));
-        const int bz0 = fp_band_rows(p.vol_x);
+        const int bzmax = fp_band_rows(p.vol_x);
+        const int bz0 = fp_balanced_rows(bzmax, p.vol_z);
         const bool has_bp = p.subset_off[1] > p.subset_off[0];
         float best_bp = 1e30f;
         const int shapes[4][2] = {{16, 16}, {16, 8}, {8, 16}, {8, 8}};
@@ -653,7 +656,8 @@ static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_
             }
         }
         float best_fp = 1e30f;
-        std::vector<int> bzs = {bz0, std::max(2, bz0 * 2 / 3), std::max(2, bz0 / 2)};
+        std::vector<int> bzs = {bz0, fp_balanced_rows(std::max(2, bzmax * 2 / 3), p.vol_z),
+                                fp_balanced_rows(std::max(2, bzmax / 2), p.vol_z)};
         for (int bz : bzs) {
             if ((st = sart_tables(p, best.tz, best.chunk, bz)) != ATR_OK) return st;
             const float ms = time_ms(s, 3, [&] { launch_fp_diff(p, 0, s); });

@@ -263,6 +263,7 @@ void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                      int bz, Geometry& g);
 int fp_band_rows(int vol_x);
+int fp_balanced_rows(int bz_max, int vol_z);
 int fp_group(int proj, int n_list);
 void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, int group,
                       std::vector<int>& tab);

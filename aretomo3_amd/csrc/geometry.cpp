@@ -351,6 +351,16 @@ int fp_band_rows(int vol_x)
     return std::min(FP_BZ_MAX, std::max(1, rows - 1));
 }
 
+// The VolZ - 1 cell rows split into equal bands of at most bz_max rows: a
+// short last band costs a full band's reload and per-pair overhead.
+int fp_balanced_rows(int bz_max, int vol_z)
+{
+    const int cells = std::max(1, vol_z - 1);
+    bz_max = std::max(1, bz_max);
+    const int nb = (cells + bz_max - 1) / bz_max;
+    return (cells + nb - 1) / nb;
+}
+
 // (tilt-in-group, tile) of every pair index p = wave + 16 i of the forward
 // projector: j << 16 | tile.
 // Tilts per forward-projection workgroup for a list of n_list tilts: at most
