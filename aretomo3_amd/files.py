@@ -6,9 +6,10 @@
   CSaveAlignFile.cpp:67-127), through ``atr_aln_*``.
 * The ``_TLT.txt`` / ``.rawtlt`` tilt-angle file
   (DataUtil/CTsPackage.cpp:358-388), through ``atr_tilt_file_read``.
-* :func:`reconstruct_files`: raw MRC + ``.aln`` -> volume MRC, the
-  reconstruction part of CAreTomoMain::mSkipAlign (AreTomo/CAreTomoMain.cpp:138-171)
-  with mRecon (:725-761) and mSartRecon / mWbpRecon (:809-879).
+* :func:`reconstruct_all` / :func:`reconstruct_files`: raw MRC + ``.aln`` ->
+  volume MRCs, the reconstruction part of CAreTomoMain::mSkipAlign
+  (AreTomo/CAreTomoMain.cpp:138-171) with mRecon (:725-761), mRecon2nd
+  (:763-797) and mSartRecon / mWbpRecon (:809-879).
 
 Parsing and writing run in the C library; this module only moves arrays.
 """
@@ -168,30 +169,22 @@ def binned_vol_z(vol_z, thickness, ext_z=300, at_bin=1.0):
     return max(z, 100)
 
 
-def reconstruct_files(mrc_path, aln_path, out_stem, vol_z=None, ext_z=300, at_bin=1.0, wbp=False,
-                      sart=(20, 3), recon_range=(-90.0, 90.0), flip_vol=0, pixel_size=None,
-                      storage=L.STORAGE_F32, fill_seed=0, write=True):
-    """Raw tilt series (MRC) + alignment (.aln) -> reconstructed volume.
+def second_vol_z(vol_z, thickness, ext_z, at_bin):
+    """CAreTomoMain::mRecon2nd (AreTomo/CAreTomoMain.cpp:775-790): VolZ from
+    -VolZ or thickness + ExtZ, at least 100, then divided by the binning and
+    made even (no lower bound after binning; at most 16 skips the volume)."""
+    z = vol_z if vol_z is not None and vol_z > 0 else thickness + ext_z
+    if z <= 100:
+        z = 100
+    return int(np.float32(z) / np.float32(at_bin)) // 2 * 2
 
-    Images are taken by the .aln section indices, so dark frames listed there
-    are left out (CRemoveDarkFrames).  The pre-stage applies the per-tilt
-    tilt axis, shifts and local patch shifts, -AtBin Fourier-bins, then WBP
-    or SART reconstructs; the volume is written as ``out_stem + _Vol.mrc``
-    in the -FlipVol layout.  Returns (volume, path or None)."""
-    aln = read_aln(aln_path)
-    info = mrc_info(mrc_path)
-    if info["nz"] != aln.raw_size[2]:
-        raise L.AtrError(L.ATR_ERR_INVALID_ARG,
-                         f"{mrc_path} has {info['nz']} sections, the .aln expects {aln.raw_size[2]}")
-    raw = read_mrc(mrc_path)
-    proj = np.ascontiguousarray(raw[aln.sec - 1])
+
+def _recon(proj, aln, vol_z, at_bin, wbp, sart, recon_range, layout, storage, fill_seed):
     n, ry, rx = proj.shape
-    z = binned_vol_z(vol_z, aln.thickness, ext_z, at_bin)
-    layout = {0: L.LAYOUT_XZY, 1: L.LAYOUT_XYZ, 2: L.LAYOUT_XYZ_NOFLIP}[int(flip_vol)]
     kw = dict(tilt_axis=aln.tilt_axis, shifts=aln.shifts, local_params=aln.local, bin=at_bin,
               layout=layout, storage=storage, fill_seed=fill_seed)
     if wbp:
-        plan = Plan(rx, ry, aln.tilts, z, L.METHOD_WBP, **kw)
+        plan = Plan(rx, ry, aln.tilts, vol_z, L.METHOD_WBP, **kw)
     else:
         # CAreTomoMain::mSartRecon (:813-826): tilt range from -ReconRange,
         # subsets = tilts / (-Sart second value)
@@ -201,24 +194,88 @@ def reconstruct_files(mrc_path, aln_path, out_stem, vol_z=None, ext_z=300, at_bi
             raise L.AtrError(L.ATR_ERR_INVALID_ARG, "empty SART tilt range")
         n_sart = end - start + 1
         subsets = max(1, n_sart // int(sart[1]))
-        plan = Plan(rx, ry, aln.tilts, z, L.METHOD_SART, sart_iters=int(sart[0]),
+        plan = Plan(rx, ry, aln.tilts, vol_z, L.METHOD_SART, sart_iters=int(sart[0]),
                     sart_subsets=subsets, sart_start_tilt=start, sart_n_tilts=n_sart, **kw)
     try:
-        vol = plan.execute_host(proj)
-        vol_y = plan.vol_y
+        return plan.execute_host(proj), plan.vol_y
     finally:
         plan.close()
-    path = None
-    if write:
-        pix = info["pixel_size"] if pixel_size is None else float(pixel_size)
-        if at_bin != 1.0:
-            # CBinStack::DoFFT (AreTomo/Correct/CBinStack.cpp:66-67):
-            # (pixel / binned height) * aligned height, in float
-            _, ay = aligned_size(rx, ry, aln.tilt_axis[0])
-            pix = float(np.float32(np.float32(pix) / np.float32(vol_y)) * np.float32(ay))
-        path = os.fspath(out_stem) + volume_suffix(0)
-        write_mrc(path, vol, pix)
-    return vol, path
+
+
+def _load_selected(mrc_path, aln):
+    info = mrc_info(mrc_path)
+    if info["nz"] != aln.raw_size[2]:
+        raise L.AtrError(L.ATR_ERR_INVALID_ARG,
+                         f"{mrc_path} has {info['nz']} sections, the .aln expects {aln.raw_size[2]}")
+    raw = read_mrc(mrc_path)
+    return np.ascontiguousarray(raw[aln.sec - 1]), info
+
+
+def reconstruct_all(mrc_path, aln_path, out_stem, vol_z=None, ext_z=300, at_bin=(1.0, 0.0, 0.0),
+                    wbp=False, sart=(20, 3), recon_range=(-90.0, 90.0), flip_vol=0,
+                    pixel_size=None, storage=L.STORAGE_F32, fill_seed=0, evn_mrc=None,
+                    odd_mrc=None, write=True):
+    """Every volume AreTomo3 ``-Cmd 2`` writes from a raw series and its .aln.
+
+    * ``_Vol``: the series at ``at_bin[0]`` by WBP or SART (``wbp``),
+      CAreTomoMain::mRecon (AreTomo/CAreTomoMain.cpp:725-761);
+    * ``_EVN_Vol`` / ``_ODD_Vol``: the even / odd frame-sum series
+      (``evn_mrc``, ``odd_mrc``, -SplitSum) with the same alignment and method;
+    * ``_2ND_Vol``: WBP at ``at_bin[1]`` and ``_3RD_Vol``: SART at
+      ``at_bin[2]`` when those are >= 1, CAreTomoMain::mRecon2nd (:763-797).
+
+    Images are taken by the .aln section indices, so dark frames listed there
+    are left out (CRemoveDarkFrames).  The pre-stage applies the per-tilt tilt
+    axis, shifts and local patch shifts, -AtBin Fourier-bins, then the
+    reconstruction runs; volumes are written as ``out_stem + suffix`` in the
+    -FlipVol layout with the binned pixel size.  Returns {suffix index:
+    (volume, path or None)} (0 _Vol, 1 _EVN_Vol, 2 _ODD_Vol, 3 _2ND_Vol,
+    4 _3RD_Vol)."""
+    aln = read_aln(aln_path)
+    bins = tuple(at_bin) if np.ndim(at_bin) else (float(at_bin), 0.0, 0.0)
+    bins = tuple(bins) + (0.0,) * (3 - len(bins))
+    layout = {0: L.LAYOUT_XZY, 1: L.LAYOUT_XYZ, 2: L.LAYOUT_XYZ_NOFLIP}[int(flip_vol)]
+    proj, info = _load_selected(mrc_path, aln)
+    n, ry, rx = proj.shape
+    pix0 = info["pixel_size"] if pixel_size is None else float(pixel_size)
+    _, ay = aligned_size(rx, ry, aln.tilt_axis[0])
+    jobs = []  # (suffix index, projections, vol_z, bin, wbp)
+    z = binned_vol_z(vol_z, aln.thickness, ext_z, bins[0])
+    jobs.append((0, proj, z, bins[0], wbp))
+    for idx, path in ((1, evn_mrc), (2, odd_mrc)):
+        if path is not None:
+            jobs.append((idx, _load_selected(path, aln)[0], z, bins[0], wbp))
+    for idx, b, use_wbp in ((3, bins[1], True), (4, bins[2], False)):
+        if b >= 1.0:
+            z2 = second_vol_z(vol_z, aln.thickness, ext_z, b)
+            if z2 > 16:
+                jobs.append((idx, proj, z2, b, use_wbp))
+    out = {}
+    for idx, p, zz, b, use_wbp in jobs:
+        vol, vol_y = _recon(p, aln, zz, b, use_wbp, sart, recon_range, layout, storage, fill_seed)
+        path = None
+        if write:
+            pix = pix0
+            if b != 1.0:
+                # CBinStack::DoFFT (AreTomo/Correct/CBinStack.cpp:66-67):
+                # (pixel / binned height) * aligned height, in float
+                pix = float(np.float32(np.float32(pix0) / np.float32(vol_y)) * np.float32(ay))
+            path = os.fspath(out_stem) + volume_suffix(idx)
+            write_mrc(path, vol, pix)
+        out[idx] = (vol, path)
+    return out
+
+
+def reconstruct_files(mrc_path, aln_path, out_stem, vol_z=None, ext_z=300, at_bin=1.0, wbp=False,
+                      sart=(20, 3), recon_range=(-90.0, 90.0), flip_vol=0, pixel_size=None,
+                      storage=L.STORAGE_F32, fill_seed=0, write=True):
+    """The main volume of :func:`reconstruct_all` (``_Vol.mrc``):
+    returns (volume, path or None)."""
+    res = reconstruct_all(mrc_path, aln_path, out_stem, vol_z=vol_z, ext_z=ext_z,
+                          at_bin=(float(at_bin), 0.0, 0.0), wbp=wbp, sart=sart,
+                          recon_range=recon_range, flip_vol=flip_vol, pixel_size=pixel_size,
+                          storage=storage, fill_seed=fill_seed, write=write)
+    return res[0]
 
 
 def aligned_size(raw_x, raw_y, tilt_axis_deg):

@@ -236,3 +236,12 @@ def test_driver_helpers():
     assert F.binned_vol_z(-1, 60, 0, 1.0) == 100       # at least 100
     assert F.binned_vol_z(301, 0, 300, 2.0) == 150
     assert F.aligned_size(64, 48, 85.0) == (48, 64)
+
+
+def test_second_resolution_vol_z():
+    # mRecon2nd: VolZ or thickness + ExtZ, at least 100 before binning,
+    # then /bin and even, no floor after binning
+    assert F.second_vol_z(None, 150, 300, 4.0) == 112
+    assert F.second_vol_z(50, 0, 0, 8.0) == 12          # 100 / 8 -> 12 (skipped by the driver: <= 16)
+    assert F.second_vol_z(0, 40, 20, 2.0) == 50         # 60 -> 100 -> 50
+    assert F.second_vol_z(301, 0, 0, 3.0) == 100

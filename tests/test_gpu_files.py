@@ -78,3 +78,40 @@ def test_reconstruct_files_matches_oracle(gpu_available, tmp_path, wbp, flip, at
     want = np.float32(np.float32(1.7) / np.float32(by)) * np.float32(ay) if at_bin != 1.0 else 1.7
     assert info["pixel_size"] == pytest.approx(float(want), rel=1e-6)
     assert path.endswith("ts_Vol.mrc")
+
+
+def test_reconstruct_all_volumes(gpu_available, tmp_path):
+    # -AtBin 2 3 4 with even/odd sums: _Vol (SART, bin 2), _EVN_Vol and
+    # _ODD_Vol (same alignment), _2ND_Vol (WBP, bin 3), _3RD_Vol (SART, bin 4)
+    raw, aln = write_inputs(tmp_path)
+    rng = np.random.default_rng(9)
+    n = len(aln.tilts)
+    full = F.read_mrc(tmp_path / "ts.mrc")
+    evn = (full * 0.5 + rng.normal(0, 0.01, full.shape)).astype(np.float32)
+    odd = (full - evn).astype(np.float32)
+    F.write_mrc(tmp_path / "ts_EVN.mrc", evn, 1.7)
+    F.write_mrc(tmp_path / "ts_ODD.mrc", odd, 1.7)
+    print("case files all volumes", flush=True)
+    res = F.reconstruct_all(tmp_path / "ts.mrc", tmp_path / "ts.aln", tmp_path / "ts", vol_z=60,
+                            ext_z=0, at_bin=(2.0, 3.0, 4.0), sart=(2, 4), flip_vol=1,
+                            evn_mrc=tmp_path / "ts_EVN.mrc", odd_mrc=tmp_path / "ts_ODD.mrc")
+    assert sorted(res) == [0, 1, 2, 3, 4]
+    a = F.read_aln(tmp_path / "ts.aln")
+    subsets = max(1, n // 4)
+    cases = {0: (full, 2.0, F.binned_vol_z(60, 30, 0, 2.0), 1),
+             1: (evn, 2.0, F.binned_vol_z(60, 30, 0, 2.0), 1),
+             2: (odd, 2.0, F.binned_vol_z(60, 30, 0, 2.0), 1),
+             3: (full, 3.0, F.second_vol_z(60, 30, 0, 3.0), 0),
+             4: (full, 4.0, F.second_vol_z(60, 30, 0, 4.0), 1)}
+    for idx, (stack, b, z, method) in cases.items():
+        sel = stack[a.sec - 1]
+        aligned = oracle.bin_stack(oracle.prestage(sel, a.tilt_axis, a.shifts, 1, 0, local=a.local), b)
+        ref = oracle.recon_slices(aligned, a.tilts, method, z, list(range(aligned.shape[1])),
+                                  iters=2, n_subsets=subsets)
+        ref = oracle.flip_vol(ref, 1)
+        vol, path = res[idx]
+        err = rel_l2(vol, ref)
+        print(idx, path, vol.shape, "rel_l2", err)
+        assert vol.shape == ref.shape and err <= TOL32
+        assert path.endswith("ts" + F.volume_suffix(idx))
+        np.testing.assert_array_equal(F.read_mrc(path), vol)
