@@ -232,6 +232,9 @@ def reconstruct_all(mrc_path, aln_path, out_stem, vol_z=None, ext_z=300, at_bin=
     (volume, path or None)} (0 _Vol, 1 _EVN_Vol, 2 _ODD_Vol, 3 _2ND_Vol,
     4 _3RD_Vol)."""
     aln = read_aln(aln_path)
+    if mrc_info(mrc_path)["nz"] <= 5:
+        # CAreTomoMain::mCheckTiltSeries (AreTomo/CAreTomoMain.cpp:986-995)
+        raise L.AtrError(L.ATR_ERR_INVALID_ARG, f"{mrc_path}: too few tilt images (<= 5)")
     bins = tuple(at_bin) if np.ndim(at_bin) else (float(at_bin), 0.0, 0.0)
     bins = tuple(bins) + (0.0,) * (3 - len(bins))
     layout = {0: L.LAYOUT_XZY, 1: L.LAYOUT_XYZ, 2: L.LAYOUT_XYZ_NOFLIP}[int(flip_vol)]

@@ -245,3 +245,18 @@ def test_second_resolution_vol_z():
     assert F.second_vol_z(50, 0, 0, 8.0) == 12          # 100 / 8 -> 12 (skipped by the driver: <= 16)
     assert F.second_vol_z(0, 40, 20, 2.0) == 50         # 60 -> 100 -> 50
     assert F.second_vol_z(301, 0, 0, 3.0) == 100
+
+
+def test_reconstruct_refuses_before_touching_a_device(tmp_path):
+    # too few tilt images (CAreTomoMain::mCheckTiltSeries), a stack that does
+    # not match the .aln, and a missing .aln all fail on the host
+    a = sample_alignment(0)
+    F.write_aln(tmp_path / "ts.aln", a)
+    F.write_mrc(tmp_path / "few.mrc", np.zeros((5, 48, 64), np.float32))
+    with pytest.raises(L.AtrError, match="too few"):
+        F.reconstruct_all(tmp_path / "few.mrc", tmp_path / "ts.aln", tmp_path / "out")
+    F.write_mrc(tmp_path / "ts.mrc", np.zeros((7, 48, 64), np.float32))   # .aln says 8
+    with pytest.raises(L.AtrError, match="sections"):
+        F.reconstruct_all(tmp_path / "ts.mrc", tmp_path / "ts.aln", tmp_path / "out")
+    with pytest.raises(L.AtrError):
+        F.reconstruct_all(tmp_path / "ts.mrc", tmp_path / "none.aln", tmp_path / "out")

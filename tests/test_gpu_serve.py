@@ -108,3 +108,25 @@ def test_serving_budget_admission(gpu_available):
     with pytest.raises(L.AtrError):
         srv.submit(raw, tilts, vz, **kw)
     srv.close()
+
+
+def test_serving_isolates_a_failed_request(gpu_available):
+    # a request whose plan cannot be built (a 95-degree tilt: no forward-ray
+    # geometry) fails alone, on the host, before any kernel runs; the requests
+    # around it complete and match their stand-alone volumes
+    good = [series(r) for r in (REQS[0], REQS[3])]
+    raw, tilts, vz, kw, _ = series(REQS[3])
+    bad_tilts = tilts.copy()
+    bad_tilts[-1] = 95.0
+    print("case serving failure isolation", flush=True)
+    srv = Server(storage=L.STORAGE_F16, max_active=2)
+    ids = [srv.submit(good[0][0], good[0][1], good[0][2], **good[0][3]),
+           srv.submit(raw, bad_tilts, vz, **kw),
+           srv.submit(good[1][0], good[1][1], good[1][2], **good[1][3])]
+    vol0, _, info0 = srv.wait(ids[0])
+    with pytest.raises(L.AtrError, match="SART needs every tilt"):
+        srv.wait(ids[1])
+    vol2, _, info2 = srv.wait(ids[2])
+    srv.close()
+    for (r, t, z, k, _), vol in zip(good, (vol0, vol2)):
+        assert rel_l2(vol, alone(r, t, z, k, L.METHOD_SART, L.STORAGE_F16)) <= 1e-6
