@@ -21,7 +21,9 @@ wbp2048-f16 / wbp2048-bf16 (2048^2 x 61 tilts, volume 2048x2048x512, 2000
 ellipsoids of semi-axes 4-80, noise 10%, 10 SART iterations, projections
 stored in FP16 / BF16 with FP32 accumulation), serve (the serving loop: a
 Poisson stream of mixed-size WBP + SART requests, reporting p50/p99 latency and
-tomograms/s beside the plain per-series FP32 path), small (a reduced SART case
+tomograms/s beside the plain per-series FP32 path), batch16 (16 uneven series:
+WBP each, then SART of all 16 in persistent batched launches replayed from a
+HIP graph, beside the same work series by series), small (a reduced SART case
 for quick checks; not a benchmark).
 
 Prints one JSON line.  Options: --steps K --warmup W --gpus N (only 1),
@@ -271,12 +273,130 @@ def serve_bench(args):
     print(json.dumps(result))
 
 
+BATCH = dict(series=16, seed=20261205, iters=15, subsets=5)
+
+
+def batch_bench(args):
+    """The batching configuration: 16 seeded series of uneven sizes (tilts
+    {31,41,51,61} over +-60 deg, projections {512,768,1024}^2, thickness
+    {128,256,384}, tilt axis uniform 80-95 deg + shifts +-3 px, 200-800
+    ellipsoids, noise 10%), FP32 projection storage.  A step is every series'
+    WBP, then the SART of all 16 (15 iterations x 5 interleaved subsets, relax
+    0.3, positivity off) as one persistent launch per SART pass, the passes
+    replayed from a HIP graph; forward-projection band rows per series from
+    the autotuner.  Reported beside the same work run series by series."""
+    from aretomo3_amd import Batch, DeviceArray, Plan, synchronize
+    from aretomo3_amd import _lib as L
+    from aretomo3_amd import phantom
+    c = dict(BATCH)
+    rng = np.random.default_rng(c["seed"])
+    sart_plans, wbp_plans, d_in, d_s, d_w, updates, shapes = [], [], [], [], [], 0.0, []
+    t_gen = time.time()
+    for k in range(c["series"]):
+        n = int(rng.choice([31, 41, 51, 61]))
+        px = int(rng.choice([512, 768, 1024]))
+        vz = int(rng.choice([128, 256, 384]))
+        axis = float(rng.uniform(80.0, 95.0))
+        tilts = np.linspace(-60.0, 60.0, n).astype(np.float32)
+        raw, shifts = phantom.tilt_series(c["seed"] + k, px, px, tilts, vz,
+                                          n_ellipsoids=int(rng.integers(200, 801)), axis_min=4.0,
+                                          axis_max=40.0, tilt_axis=axis, max_shift=3.0,
+                                          noise_frac=0.10, n_threads=16)
+        geo = dict(tilt_axis=np.full(n, axis, np.float32), shifts=shifts, fill_seed=c["seed"] + k)
+        ps = Plan(px, px, tilts, vz, L.METHOD_SART, sart_iters=c["iters"], sart_subsets=c["subsets"],
+                  relax=0.3, subset_order=L.SUBSETS_INTERLEAVED, positivity=False, **geo)
+        if not args.no_tune:
+            tz, ch, rows = ps.autotune()
+            ps.set_tile_shape(16, 16, rows)   # a batch shares the default back-projection shape
+        pw = Plan(px, px, tilts, vz, L.METHOD_WBP, **geo)
+        sart_plans.append(ps)
+        wbp_plans.append(pw)
+        d_in.append(DeviceArray.from_numpy(raw))
+        d_s.append(DeviceArray(4 * ps.vol_x * ps.vol_y * ps.vol_z))
+        d_w.append(DeviceArray(4 * pw.vol_x * pw.vol_y * pw.vol_z))
+        vox = float(ps.vol_x) * ps.vol_y * ps.vol_z
+        updates += vox * n * (1 + 1 + 2 * c["iters"])   # WBP + SART passes
+        shapes.append((n, px, vz))
+        del raw
+    print(f"[bench] batch16: {len(shapes)} series {shapes} generated in {time.time() - t_gen:.1f}s",
+          file=sys.stderr, flush=True)
+
+    def run_plain():
+        for k in range(len(sart_plans)):
+            wbp_plans[k].execute(d_in[k], d_w[k])
+            sart_plans[k].execute(d_in[k], d_s[k])
+
+    batch = Batch(sart_plans, graph=True)
+
+    def run_batch():
+        for k in range(len(wbp_plans)):
+            wbp_plans[k].execute(d_in[k], d_w[k])
+        batch.execute(d_in, d_s)
+
+    run_plain()
+    synchronize()
+    t0 = time.perf_counter()
+    run_plain()
+    synchronize()
+    t_plain = time.perf_counter() - t0
+    ref = [d_s[k].to_numpy(sart_plans[k].volume_shape) for k in (0, len(sart_plans) - 1)]
+    for _ in range(max(args.warmup, 1)):
+        run_batch()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_batch()
+    synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    errs = []
+    for ref_k, k in zip(ref, (0, len(sart_plans) - 1)):
+        got = d_s[k].to_numpy(sart_plans[k].volume_shape)
+        errs.append(float(np.linalg.norm((got - ref_k).ravel()) / np.linalg.norm(ref_k.ravel())))
+    batch.close()
+    for p_ in sart_plans + wbp_plans:
+        p_.close()
+    print(f"[bench] batch16: {args.steps} steps, {wall:.3f} s/step (plain {t_plain:.3f} s)",
+          file=sys.stderr, flush=True)
+    result = {
+        "metric": METRIC,
+        "value": round(updates / wall / 1e9, 2),
+        "unit": "Gvoxel-tilt updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": round(1e3 * wall, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantoms, analytic projections)",
+        "config": {
+            "workload": ("batch16: 16 series (tilts {31,41,51,61} over +-60 deg, projections "
+                         "{512,768,1024}^2, thickness {128,256,384}, tilt axis 80-95 deg + shifts "
+                         "+-3 px, 200-800 ellipsoids, noise 10%%), per step WBP of each series then "
+                         "SART %d iterations x %d interleaved subsets (relax 0.3, positivity off) of "
+                         "all 16 in persistent batched launches replayed from a HIP graph, FP32 "
+                         "projection storage" % (c["iters"], c["subsets"])),
+            "series": [{"tilts": n, "projection": px, "thickness": vz} for n, px, vz in shapes],
+            "updates_per_step": updates,
+            "projection_storage": "f32",
+        },
+        "batching": {
+            "plain_per_series_s": round(t_plain, 3),
+            "batched_s": round(wall, 3),
+            "speedup_vs_plain": round(t_plain / wall, 3),
+            "batched_vs_plain_rel_l2_max": max(errs),
+        },
+    }
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="sart1024", choices=sorted(CONFIGS) + ["serve"])
+    ap.add_argument("--config", default="sart1024", choices=sorted(CONFIGS) + ["serve", "batch16"])
     ap.add_argument("--requests", type=int, default=0, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second")
     ap.add_argument("--small", action="store_true", help="same as --config small")
@@ -297,6 +417,9 @@ def main():
         sys.exit(2)
     if args.config == "serve":
         serve_bench(args)
+        return
+    if args.config == "batch16":
+        batch_bench(args)
         return
     name = "small" if args.small else args.config
     w = CONFIGS[name]
