@@ -23,8 +23,10 @@ stored in FP16 / BF16 with FP32 accumulation), serve (the serving loop: a
 Poisson stream of mixed-size WBP + SART requests, reporting p50/p99 latency and
 tomograms/s beside the plain per-series FP32 path), batch16 (16 uneven series:
 WBP each, then SART of all 16 in persistent batched launches replayed from a
-HIP graph, beside the same work series by series), small (a reduced SART case
-for quick checks; not a benchmark).
+HIP graph, beside the same work series by series), extras (the pre-stage with
+local shifts and binning, the re-projector, the thickness measurement and MRC
+I/O, each beside the FP64 oracle), small (a reduced SART case for quick checks;
+not a benchmark).
 
 Prints one JSON line.  Options: --steps K --warmup W --gpus N (only 1),
 --config NAME, --no-cpu-baseline, --cpu-slices S, --copy-probe MIB,
@@ -391,12 +393,181 @@ def batch_bench(args):
     print(json.dumps(result))
 
 
+def extras_bench(args):
+    """Measurements of the steps either side of the reconstruction, each with
+    its algorithmic bytes per call against the HBM peak and the FP64 oracle on
+    the host cores as the CPU baseline (bounded samples, scaled):
+
+    * pre-stage with local patch shifts and -AtBin 4 Fourier binning of a
+      4096^2 x 41 series (tilt axis 86 deg, shifts +-3 px, 25 patches);
+    * the ProjAlign re-projector on a 1024^2 x 41 stack, volume 256 thick,
+      every projection predicted in projection-matching order;
+    * the sample-thickness measurement (CCalcVolThick) of a 2048^2 x 41
+      series, binning 20;
+    * MRC write and read of a 1 GiB volume (host I/O, no GPU)."""
+    import tempfile
+
+    import oracle
+    from aretomo3_amd import CalcVolThick, DeviceArray, Plan, files, phantom, synchronize
+    from aretomo3_amd import _lib as L
+    from aretomo3_amd.projalign import CalcReproj, find_proj_range
+    out = {}
+    threads = oracle.max_threads()
+    rng = np.random.default_rng(20261207)
+
+    # ---- pre-stage: local shifts + Fourier binning ----
+    px, n, binv, npat = 4096, 41, 4.0, 25
+    tilts = phantom.tilt_range(-60.0, 60.0, 3.0).astype(np.float32)
+    raw, shifts = phantom.tilt_series(20261207, px, px, tilts, 512, n_ellipsoids=300, axis_min=8.0,
+                                      axis_max=80.0, tilt_axis=86.0, max_shift=3.0, noise_frac=0.1,
+                                      n_threads=16)
+    g = np.linspace(-0.4, 0.4, 5) * px
+    local = np.empty((n, 5, npat), np.float32)
+    local[:, 0] = np.tile(np.repeat(g, 5), (n, 1))
+    local[:, 1] = np.tile(np.tile(g, 5), (n, 1))
+    local[:, 2:4] = rng.uniform(-2, 2, (n, 2, npat))
+    local[:, 4] = 1.0
+    axes = np.full(n, 86.0, np.float32)
+    plan = Plan(px, px, tilts, 128, L.METHOD_WBP, tilt_axis=axes, shifts=shifts, local_params=local,
+                bin=binv, fill_seed=7)
+    d_raw = DeviceArray.from_numpy(raw)
+    d_vol = DeviceArray(4 * plan.vol_x * plan.vol_y * plan.vol_z)
+    plan.execute(d_raw, d_vol)
+    synchronize()
+    plan.reset_stats()
+    plan.set_profiling(True)
+    reps = 3
+    for _ in range(reps):
+        plan.execute(d_raw, d_vol)
+    synchronize()
+    nl, ms = plan.kernel_stats(L.K_PRESTAGE)
+    plan.set_profiling(False)
+    ms /= reps
+    ax, ay = files.aligned_size(px, px, 86.0)
+    bx, by = plan.vol_x, plan.vol_y
+    kx = bx // 2 + 1
+    bytes_pre = n * (4.0 * px * px + 4.0 * ax * ay + 4.0 * ax * ay + 16.0 * ay * kx + 16.0 * by * kx
+                     + 4.0 * bx * by)
+    plan.close()
+    d_vol.free()
+    t0 = time.perf_counter()
+    k_cpu = 2
+    al = oracle.prestage(raw[:k_cpu], axes[:k_cpu], shifts[:k_cpu], 1, 7, local=local[:k_cpu])
+    oracle.bin_stack(al, binv)
+    t_cpu = (time.perf_counter() - t0) / k_cpu
+    out["prestage_local_bin"] = {
+        "workload": "%d^2 x %d raw, tilt axis 86 deg + shifts +-3 px, %d local patches, "
+                    "Fourier binning %g -> %dx%d" % (px, n, npat, binv, bx, by),
+        "ms_per_series": round(ms, 3), "tilts_per_s": round(n / (ms * 1e-3), 1),
+        "algorithmic_bytes": bytes_pre, "achieved_gbs": round(bytes_pre / (ms * 1e-3) / 1e9, 1),
+        "hbm_frac": round(bytes_pre / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "cpu_baseline": {"tilts_per_s": round(1.0 / t_cpu, 3), "cores": threads, "kind": "port",
+                         "sample": "FP64 oracle pre-stage + binning of %d tilts, %.1fs" % (k_cpu, t_cpu * k_cpu)},
+    }
+    del raw, d_raw, al
+    print(f"[bench] extras prestage: {out['prestage_local_bin']}", file=sys.stderr, flush=True)
+
+    # ---- ProjAlign re-projector ----
+    px, vz = 1024, 256
+    projs, _ = phantom.tilt_series(20261208, px, px, tilts, vz, n_ellipsoids=200, axis_min=4.0,
+                                   axis_max=40.0, noise_frac=0.1, n_threads=16)
+    rp = CalcReproj(px, px, n, vz)
+    d_p = DeviceArray.from_numpy(projs)
+    d_o = DeviceArray(4 * px * px)
+    order = np.argsort(np.abs(tilts), kind="stable")[1:]
+    jobs = []
+    for idx in order:
+        skip = (np.abs(tilts) >= abs(tilts[idx]) - 1e-6).astype(np.uint8)
+        try:
+            jobs.append((int(idx), skip, find_proj_range(tilts, idx, skip)))
+        except L.AtrError:
+            pass
+    rp.execute(d_p, tilts, jobs[0][1], jobs[0][0], d_o)
+    synchronize()
+    t0 = time.perf_counter()
+    for idx, skip, _ in jobs:
+        rp.execute(d_p, tilts, skip, idx, d_o)
+    synchronize()
+    ms_r = 1e3 * (time.perf_counter() - t0) / len(jobs)
+    nr = np.mean([b - a + 1 for _, _, (a, b) in jobs])
+    bytes_r = 4.0 * nr * px * px + 2 * 4.0 * (2 * px) * vz * px + 4.0 * px * px
+    rows = 8
+    sub = np.ascontiguousarray(projs[:, px // 2 - rows // 2: px // 2 + rows // 2])
+    idx, skip, _ = jobs[len(jobs) // 2]
+    t0 = time.perf_counter()
+    oracle.reproj(sub, tilts, vz, idx, skip)
+    t_cpu = (time.perf_counter() - t0) * (px / rows)
+    rp.close()
+    out["reproj"] = {
+        "workload": "%d^2 x %d stack, volume thickness %d, %d projections predicted "
+                    "(projection-matching order, mean %.1f reference tilts)" % (px, n, vz, len(jobs), nr),
+        "ms_per_reprojection": round(ms_r, 3),
+        "algorithmic_bytes": bytes_r, "achieved_gbs": round(bytes_r / (ms_r * 1e-3) / 1e9, 1),
+        "hbm_frac": round(bytes_r / (ms_r * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "cpu_baseline": {"ms_per_reprojection": round(1e3 * t_cpu, 1), "cores": threads, "kind": "port",
+                         "sample": "FP64 oracle on %d of %d rows, scaled" % (rows, px)},
+    }
+    del projs, d_p, d_o
+    print(f"[bench] extras reproj: {out['reproj']}", file=sys.stderr, flush=True)
+
+    # ---- sample thickness ----
+    px = 2048
+    raw, shifts = phantom.tilt_series(20261209, px, px, tilts, 400, n_ellipsoids=400, axis_min=10.0,
+                                      axis_max=60.0, tilt_axis=86.0, max_shift=3.0, noise_frac=0.1,
+                                      n_threads=16)
+    c = CalcVolThick()
+    c.do_it(raw, tilts, 1.0, tilt_axis=axes, shifts=shifts, fill_seed=3)
+    t0 = time.perf_counter()
+    got = c.do_it(raw, tilts, 1.0, tilt_axis=axes, shifts=shifts, fill_seed=3)
+    ms_t = 1e3 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    ref = oracle.calc_vol_thick(raw, tilts, axes, shifts, 1.0, seed=3)
+    t_cpu = time.perf_counter() - t0
+    out["thickness"] = {
+        "workload": "%d^2 x %d raw, pixel 1 A (binning 20), sample slab 400 px" % (px, n),
+        "ms_per_series": round(ms_t, 2), "thickness_px": got["thickness"],
+        "same_edges_as_oracle": (got["low_edge"], got["high_edge"]) == ref["edges"][:2],
+        "note": "host call: includes the H2D copy of the raw series",
+        "cpu_baseline": {"ms_per_series": round(1e3 * t_cpu, 1), "cores": threads, "kind": "port",
+                         "sample": "FP64 oracle pipeline on the same series"},
+    }
+    del raw
+    print(f"[bench] extras thickness: {out['thickness']}", file=sys.stderr, flush=True)
+
+    # ---- MRC I/O ----
+    vol = np.random.default_rng(1).standard_normal((256, 1024, 1024), dtype=np.float32)
+    with tempfile.TemporaryDirectory() as d:
+        pth = os.path.join(d, "v_Vol.mrc")
+        t0 = time.perf_counter()
+        files.write_mrc(pth, vol, 1.0)
+        t_w = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        back = files.read_mrc(pth)
+        t_r = time.perf_counter() - t0
+        ok = bool(np.array_equal(back, vol))
+    out["mrc_io"] = {"bytes": vol.nbytes, "write_gbs": round(vol.nbytes / t_w / 1e9, 2),
+                     "read_gbs": round(vol.nbytes / t_r / 1e9, 2), "round_trip_identical": ok}
+    result = {
+        "metric": "pre-stage tilts/s (local shifts + Fourier binning) and per-step times of the "
+                  "re-projector, thickness measurement and MRC I/O",
+        "value": out["prestage_local_bin"]["tilts_per_s"],
+        "unit": "tilts/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": out["prestage_local_bin"]["ms_per_series"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64 geometry",
+        "data": "synthetic (seeded ellipsoid phantoms, analytic projections)",
+        "config": {"workload": "extras: pre-stage (local + bin), re-projector, thickness, MRC I/O"},
+        "extras": out,
+    }
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="sart1024", choices=sorted(CONFIGS) + ["serve", "batch16"])
+    ap.add_argument("--config", default="sart1024", choices=sorted(CONFIGS) + ["serve", "batch16", "extras"])
     ap.add_argument("--requests", type=int, default=0, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second")
     ap.add_argument("--small", action="store_true", help="same as --config small")
@@ -420,6 +591,9 @@ def main():
         return
     if args.config == "batch16":
         batch_bench(args)
+        return
+    if args.config == "extras":
+        extras_bench(args)
         return
     name = "small" if args.small else args.config
     w = CONFIGS[name]
