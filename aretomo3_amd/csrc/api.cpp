@@ -154,7 +154,9 @@ static atr_status plan_sizes(const atr_series_desc* d, Plan& p)
             return fail(ATR_ERR_INVALID_ARG, "sart_subsets must be in [1, number of SART tilts]");
         p.relax = d->sart_relax > 0 ? d->sart_relax : 1.0f / (float)p.n_subsets;
         if (2 * (p.vol_x + 1) * (int)sizeof(float2) > FP_LDS_BUDGET)
-            return fail(ATR_ERR_UNSUPPORTED_SIZE, "SART supports VolX <= 4991");
+            return fail(ATR_ERR_UNSUPPORTED_SIZE,
+                        "SART supports VolX <= " +
+                            std::to_string(FP_LDS_BUDGET / (2 * (int)sizeof(float2)) - 1));
         // subset lists (Recon/CTomoSart.cpp:60-61,74-80; AreTomo/Util/CSplitItems.cpp)
         std::vector<int> starts, sizes;
         split_items(p.sart_n, p.n_subsets, starts, sizes);

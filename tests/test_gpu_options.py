@@ -170,3 +170,24 @@ def test_larger_series_sampled_slices(gpu_available):
     err = rel_l2(gpu[sl], ref)
     print("rel_l2", err, "shape", gpu.shape)
     assert err <= TOL32
+
+
+def test_widest_supported_slices(gpu_available):
+    # SART at ProjX 4096: 3-row forward-projection bands (4097 pairs per LDS
+    # row), 64 ray tiles per tilt, one tilt per forward workgroup; WBP at its
+    # widest VolX (2048: 4096-point row FFTs on the 2x grid)
+    from aretomo3_amd import phantom
+    for px, py, vz, method, step in ((4096, 2, 40, L.METHOD_SART, 6.0),
+                                     (2048, 2, 48, L.METHOD_WBP, 6.0)):
+        tilts = phantom.tilt_range(-60, 60, step)
+        proj, _ = phantom.tilt_series(77, px, py, tilts, vz, n_ellipsoids=40, axis_min=8,
+                                      axis_max=60, noise_frac=0.05)
+        print(f"case widest px={px} method={method}", flush=True)
+        plan = Plan(px, py, tilts, vz, method, sart_iters=2, sart_subsets=3)
+        gpu = plan.execute_host(proj)
+        plan.close()
+        ref = oracle.recon_slices(proj.astype(np.float64), tilts, 1 if method == L.METHOD_SART else 0,
+                                  vz, list(range(py)), iters=2, n_subsets=3)
+        err = rel_l2(gpu, ref)
+        print("rel_l2", err)
+        assert err <= 1e-5
