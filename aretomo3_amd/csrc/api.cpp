@@ -227,7 +227,7 @@ static size_t plan_bytes_estimate(const Plan& p)
         b += (size_t)p.wbp_slab * 3 * p.vol_z * (p.vol_x / 2 + 1) * sizeof(float2);
         b += (size_t)p.wbp_slab * 4 * p.vol_z * p.vol_x * sizeof(float);
         b += n * 2 * p.vol_z * 8 + n * 16;
-        b += n * (size_t)((2 * p.vol_z + BP_TZ - 1) / BP_TZ) * ((2 * p.vol_x + BP_TX - 1) / BP_TX) * 8;
+        b += n * (size_t)((2 * p.vol_z + BP_TZ - 1) / BP_TZ) * ((2 * p.vol_x + BP_TX - 1) / BP_TX) * 16;
     }
     return b;
 }
@@ -379,7 +379,7 @@ static atr_status plan_build(Plan& p)
         {
             // 2x-grid back-projection: tile windows/classes, the tilt list
             // (excluded tilts left out, GBackProjWbp.cu:31), per-slab arguments
-            std::vector<int2> wt;
+            std::vector<int4> wt;
             build_bp_tiles(g.cos_sin, g.wbp_lohi, p.n, 2 * p.vol_x, 2 * p.vol_z, 0.5,
                            (double)(p.proj / 2) + 0.5, BP_TZ, wt);
             if ((st = upload(&p.d_wbp_tiles, wt, B)) != ATR_OK) return st;

@@ -59,12 +59,14 @@ struct TiltRow {
 struct Geometry {
     std::vector<float> cos_sin;        // [2N] float, as CTomoBase.cpp:55-65
     std::vector<int2> bp_lohi;         // [N][VolZ] x-interval of GBackProj inclusion
-    std::vector<int2> bp_tiles;        // [N][ztiles][xtiles] {window start, class 0 empty / 1 full / 2 partial}
+    // [N][ztiles][xtiles] {window start, class 0 empty / 1 full / 2 partial,
+    // FP32 bits of u - start at the tile's first voxel, 0}
+    std::vector<int4> bp_tiles;
     std::vector<int2> wbp_lohi;        // [N][2VolZ] x2-interval of GBackProjWbp inclusion
     std::vector<int2> fp_range;        // [N][P] first valid sample and count of each ray
     std::vector<int> fp_len;           // [N] ray length L
     std::vector<int> fp_geo_off;       // [N] offset into fp_geo
-    std::vector<float4> fp_geo;        // [sum L] {row index bits, wz, q, 0}
+    std::vector<float4> fp_geo;        // [sum L] {row byte offset bits, wz, q, 1 - wz}
     std::vector<int2> fp_band;         // [N][nbands] sample range whose rows lie in a z-band
     std::vector<int4> fp_desc;         // [N][P/64][nbands] first geometry entry, count, first k, full
     int fp_bz = 0, fp_nbands = 0;      // z rows per band
@@ -104,7 +106,7 @@ struct Plan {
     // device tables
     float2* d_cs = nullptr;
     int2* d_bp_lohi = nullptr;
-    int2* d_bp_tiles = nullptr;
+    int4* d_bp_tiles = nullptr;
     int2* d_wbp_lohi = nullptr;
     int2* d_fp_range = nullptr;
     int* d_fp_geo_off = nullptr;
@@ -146,7 +148,7 @@ struct Plan {
     BpArgs* d_bp_args = nullptr;      // SART step arguments: initial BP, then per subset
     FpArgs* d_fp_args = nullptr;      // per subset
     float* d_wbp_grid = nullptr;      // [slab][2VolZ][2VolX] the 2x-grid back-projection
-    int2* d_wbp_tiles = nullptr;      // [N][2VolZ/16][2VolX/256] window start, class
+    int4* d_wbp_tiles = nullptr;      // [N][2VolZ/16][2VolX/256] as bp_tiles
     float2* d_wbp_t = nullptr;        // [slab][2VolZ][VolX/2+1]
     float2* d_wbp_u = nullptr;        // [slab][VolZ][VolX/2+1]
     int wbp_slab = 0;
@@ -171,11 +173,11 @@ struct BpArgs {
     int n_list;
     const float2* cs;
     const int2* lohi;          // [N][VolZ]
-    const int2* tiles;         // [N][ztiles][xtiles] {window start, class}
+    const int4* tiles;         // [N][ztiles][xtiles] {window start, class, u0 bits, 0}
     int n_ztiles, n_xtiles;
     float relax;
     int positivity;
-    float scale, offset;       // u = scale (fx cos + fz sin) + offset
+    float scale, offset;       // u = scale (fx cos + fz sin) + offset (offset enters via the tile table)
     const int* sflag;          // NULL: the sinogram holds no empty samples
     float* vol;
     int vol_x, vol_z;
@@ -257,7 +259,7 @@ void cos_sin_table(const float* tilts, int n, std::vector<float>& cs);
 void build_bp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                         std::vector<int2>& lohi);
 void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi, int n, int vol_x,
-                    int vol_z, double scale, double offset, int tz, std::vector<int2>& tiles);
+                    int vol_z, double scale, double offset, int tz, std::vector<int4>& tiles);
 void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                          std::vector<int2>& lohi);
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,

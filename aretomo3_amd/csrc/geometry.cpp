@@ -133,12 +133,14 @@ void build_bp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol_
 // offset P/2; the WBP 2x grid: scale 1/2, offset P/2 + 1/2).
 // w0 = floor(min u over the tile corners) - 3 in FP64; the kernel's FP32
 // coordinates relative to w0 then stay in [2, window - 2] (the tile spans at
-// most scale ((BP_TX-1)|cos| + (tz-1)|sin|) in u).
+// most scale ((BP_TX-1)|cos| + (tz-1)|sin|) in u).  The third field holds
+// the FP32 bits of u - w0 at the tile's first voxel (x0, z0), formed in FP64:
+// the kernel adds (x - x0) scale cos and k scale sin to it in FP32.
 void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi, int n, int vol_x,
-                    int vol_z, double scale, double offset, int tz, std::vector<int2>& tiles)
+                    int vol_z, double scale, double offset, int tz, std::vector<int4>& tiles)
 {
     int nzt = (vol_z + tz - 1) / tz, nxt = (vol_x + BP_TX - 1) / BP_TX;
-    tiles.assign((size_t)n * nzt * nxt, int2{0, 0});
+    tiles.assign((size_t)n * nzt * nxt, int4{0, 0, 0, 0});
     for (int t = 0; t < n; t++) {
         const double c = cs[2 * t], s = cs[2 * t + 1];
         for (int zt = 0; zt < nzt; zt++)
@@ -154,8 +156,10 @@ void build_bp_tiles(const std::vector<float>& cs, const std::vector<int2>& lohi,
                 const double fz0 = zt * tz - vol_z * 0.5, fz1 = fz0 + (tz - 1);
                 const double umin =
                     offset + scale * (std::min(c * fx0, c * fx1) + std::min(s * fz0, s * fz1));
+                const int w0 = (int)std::floor(umin) - 3;
+                const float u00 = (float)(scale * (c * fx0 + s * fz0) + offset - w0);
                 tiles[((size_t)t * nzt + zt) * nxt + xt] =
-                    int2{(int)std::floor(umin) - 3, full ? 1 : (any ? 2 : 0)};
+                    int4{w0, full ? 1 : (any ? 2 : 0), __builtin_bit_cast(int, u00), 0};
             }
     }
 }
@@ -235,7 +239,8 @@ bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, 
             // byte offset of the sample's lower row inside its band buffer
             // (row iz % bz, (VolX + 1) interpolation pairs per row)
             const int row_bytes = (iz % bz) * (vol_x + 1) * 8;
-            geo[k] = make_float4(__builtin_bit_cast(float, row_bytes), wz, q, 0.0f);
+            geo[k] = make_float4(__builtin_bit_cast(float, row_bytes), wz, q,
+                                 (float)(1.0 - (zc - iz)));
             if (zin) {
                 int2& bb = g.fp_band[(size_t)t * nb + iz / bz];
                 if (bb.y < bb.x) bb = make_int2(k, k);

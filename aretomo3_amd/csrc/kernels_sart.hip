@@ -217,11 +217,11 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
     const int n_list = A->n_list;
     const cptr<float2> cs = as_cptr(A->cs);
     const cptr<int2> lohi = as_cptr(A->lohi);
-    const cptr<int2> tiles = as_cptr(A->tiles);
+    const cptr<int4> tiles = as_cptr(A->tiles);
     const int n_ztiles = A->n_ztiles, n_xtiles = A->n_xtiles;
     const float relax = A->relax;
     const int positivity = A->positivity;
-    const double scale = A->scale, offset = A->offset;
+    const float scale = A->scale;
     const cptr<int> sflag = as_cptr(A->sflag);
     const gptr<float> vol = as_gptr(A->vol);
     const int vol_x = A->vol_x, vol_z = A->vol_z;
@@ -249,10 +249,9 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
         cnt[k] = 0;
     }
     int full_cnt = 0;
-    const double fx = (double)xc - vol_x * 0.5;
-    const double fz0 = (double)z0 - vol_z * 0.5;
+    const float xf = (float)(xc - x0);   // offset from the tile's first voxel
     const gptr<const S> base = sino + (size_t)y * rows_per_slice * stride;
-    const cptr<int2> tile_tab = tiles + (size_t)zt * n_xtiles + xt;
+    const cptr<int4> tile_tab = tiles + (size_t)zt * n_xtiles + xt;
     const size_t tilt_step = (size_t)n_ztiles * n_xtiles;
 
     for (int c0 = 0; c0 < n_list; c0 += CH) {
@@ -306,13 +305,13 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
         // ---- accumulate ----
         for (int j = 0; j < nc; j++) {
             const TiltRow tr = ldc(list + c0 + j);
-            const int2 tw = ldc(tile_tab + (size_t)tr.tilt * tilt_step);   // {w0, class}
+            // {w0, class, u - w0 at the tile's first voxel}
+            const int4 tw = ldc(tile_tab + (size_t)tr.tilt * tilt_step);
             if (tw.y == 0) continue;
             const float2 sc = ldc(cs + tr.tilt);
-            const float c = sc.x, s = sc.y;
-            const double kk = scale * fz0 * (double)s + offset - (double)tw.x;
-            const float u0 = (float)(scale * fx * (double)c + kk);
-            const float ss = (float)(scale * (double)s);   // exact: scale is 1 or 1/2
+            // scale is 1 or 1/2: both products are exact
+            const float cc = scale * sc.x, ss = scale * sc.y;
+            const float u0 = fmaf(xf, cc, __int_as_float(tw.z));
             const float2* pj = pairs[j];
             if (tw.y == 1 && !sent) {
                 full_cnt += 1;
@@ -443,27 +442,48 @@ struct Geo4 {
     float4 g[4];
 };
 
-// Bilinear sample from the band: g.x is the byte offset of the sample's lower
-// row in the band buffer, xs = x + 1 (the +1 skips the guard pair), rowb the
-// byte pitch of a band row.
-__device__ __forceinline__ float fp_sample(const char* __restrict__ band, float4 g, float xs,
-                                           int rowb)
+// The two cells of a bilinear sample: g.x is the byte offset of the
+// sample's lower row in the band buffer, xs = x + 1 (the +1 skips the guard
+// pair), rowb the byte pitch of a band row.
+struct FpCells {
+    float2 e0, e1;  // (V, dV/dx) of the lower and upper row
+    float f;        // x fraction
+};
+
+__device__ __forceinline__ FpCells fp_fetch(const char* __restrict__ band, float4 g, float xs,
+                                            int rowb)
 {
     const int ix = (int)xs;
-    const float f = __builtin_amdgcn_fractf(xs);
     const char* p = band + __float_as_int(g.x) + 8 * ix;
-    const float2 e0 = *(const float2*)p;
-    const float2 e1 = *(const float2*)(p + rowb);
-    const float top = fmaf(f, e0.y, e0.x);
-    const float bot = fmaf(f, e1.y, e1.x);
+    FpCells c;
+    c.f = __builtin_amdgcn_fractf(xs);
+    c.e0 = *(const float2*)p;
+    c.e1 = *(const float2*)(p + rowb);
+    return c;
+}
+
+// Bilinear value: top + wz (bot - top), g.y = wz.
+__device__ __forceinline__ float fp_value(const FpCells& c, float4 g)
+{
+    const float top = fmaf(c.f, c.e0.y, c.e0.x);
+    const float bot = fmaf(c.f, c.e1.y, c.e1.x);
     return fmaf(g.y, bot - top, top);
+}
+
+// acc + (1 - wz) top + wz bot, g.w = 1 - wz: the unmasked form, one VALU
+// fewer than forming the value first.
+__device__ __forceinline__ float fp_accumulate(const FpCells& c, float4 g, float acc)
+{
+    const float top = fmaf(c.f, c.e0.y, c.e0.x);
+    const float bot = fmaf(c.f, c.e1.y, c.e1.x);
+    return fmaf(g.y, bot, fmaf(g.w, top, acc));
 }
 
 // Samples [0, count) of one (pair, band): geometry entries G[0..count),
 // sample indices k0 + m.  Groups of four (one 64-byte scalar load of
-// geometry), then the remainder one sample at a time.  MASKED clamps the
-// coordinate and tests each sample against the ray's FP64 interval
-// rr = (first, count).
+// geometry, the eight LDS reads issued before any is used), then the
+// remainder one sample at a time.  MASKED clamps the coordinate and tests
+// each sample against the ray's FP64 interval rr = (first, count).
 template <bool MASKED, bool SENT>
 __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
                                        cptr<float4> G, int count, int k0,
@@ -472,22 +492,26 @@ __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
     int m = 0;
     for (; m + 4 <= count; m += 4) {
         const Geo4 g4 = ldc((cptr<Geo4>)(G + m));
-        float v[4];
+        FpCells c[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             float xs = tf1 - g4.g[q].z;  // x + 1
             if (MASKED) xs = __builtin_amdgcn_fmed3f(xs, 1.0f, xmax1);
-            v[q] = fp_sample(band, g4.g[q], xs, rowb);
+            c[q] = fp_fetch(band, g4.g[q], xs, rowb);
         }
+        // keep the reads of the group ahead of their uses: scheduled next to
+        // each use, every sample would wait out a full LDS latency
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (MASKED) {
+                const float v = fp_value(c[q], g4.g[q]);
                 bool in = (unsigned)(k0 + m + q - rr.x) < (unsigned)rr.y;
-                if (SENT) in = in & !(v[q] < kEmpty);
-                acc += in ? v[q] : 0.0f;
+                if (SENT) in = in & !(v < kEmpty);
+                acc += in ? v : 0.0f;
                 if (SENT) cnt += in ? 1 : 0;
             } else {
-                acc += v[q];
+                acc = fp_accumulate(c[q], g4.g[q], acc);
             }
         }
     }
@@ -497,14 +521,15 @@ __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
         const float4 g = ldc(G + m);
         float xs = tf1 - g.z;
         if (MASKED) xs = __builtin_amdgcn_fmed3f(xs, 1.0f, xmax1);
-        const float v = fp_sample(band, g, xs, rowb);
+        const FpCells c = fp_fetch(band, g, xs, rowb);
         if (MASKED) {
+            const float v = fp_value(c, g);
             bool in = (unsigned)(k0 + m - rr.x) < (unsigned)rr.y;
             if (SENT) in = in & !(v < kEmpty);
             acc += in ? v : 0.0f;
             if (SENT) cnt += in ? 1 : 0;
         } else {
-            acc += v;
+            acc = fp_accumulate(c, g, acc);
         }
     }
 }
