@@ -484,11 +484,16 @@ __device__ __forceinline__ float fp_accumulate(const FpCells& c, float4 g, float
 // geometry, the eight LDS reads issued before any is used), then the
 // remainder one sample at a time.  MASKED clamps the coordinate and tests
 // each sample against the ray's FP64 interval rr = (first, count).
-template <bool MASKED, bool SENT>
-__device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
+//
+// ROWB > 0 is the band row pitch in bytes known at compile time: the upper
+// row's read then takes it as the LDS instruction's offset instead of an
+// address add.
+template <bool MASKED, bool SENT, int ROWB = 0>
+__device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb_rt,
                                        cptr<float4> G, int count, int k0,
                                        float tf1, float xmax1, int2 rr, float& acc, int& cnt)
 {
+    const int rowb = ROWB > 0 ? ROWB : rowb_rt;
     int m = 0;
     for (; m + 4 <= count; m += 4) {
         const Geo4 g4 = ldc((cptr<Geo4>)(G + m));
@@ -537,7 +542,7 @@ __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb,
 // One (group of tilts, slice) work item.  Returns early, for the whole
 // block, when the slice's sentinel flag does not match SENT.  Ends after a
 // barrier, with the LDS free for the next item.
-template <typename S, bool SENT>
+template <typename S, bool SENT, int ROWB = 0>
 __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ band, int gx, int y)
 {
     const gptr<const float> vol = as_gptr(A->vol);
@@ -629,7 +634,7 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
                 if (d.y > 0) {
                     const cptr<float4> G = geo + d.x;
                     if (!SENT && d.w != 0) {
-                        fp_run<false, false>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1,
+                        fp_run<false, false, ROWB>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1,
                                              make_int2(0, 0), acc[i], cnt[i]);
                     } else {
                         int xd = tile * FP_WAVE + lane;
@@ -638,7 +643,7 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
                         asm volatile("" : "+v"(xd));
                         int2 rr = make_int2(0, 0);
                         if (xd < proj) rr = ldgo(fp_range + (size_t)t * proj + xd);
-                        fp_run<true, SENT>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1, rr, acc[i],
+                        fp_run<true, SENT, ROWB>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1, rr, acc[i],
                                            cnt[i]);
                     }
                 }
@@ -665,11 +670,13 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
     }
 }
 
-template <typename S, bool SENT>
+// ROWB: the band row pitch in bytes, 8 (VolX + 1), when the launch fixes it
+// at compile time (VolX 512, 1024, 2048), else 0.
+template <typename S, bool SENT, int ROWB>
 __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(const FpArgs* __restrict__ a)
 {
     extern __shared__ float2 band[];
-    fp_block<S, SENT>(as_cptr(a), band, blockIdx.x, blockIdx.y);
+    fp_block<S, SENT, ROWB>(as_cptr(a), band, blockIdx.x, blockIdx.y);
 }
 
 // Forward projection + difference of many series in one persistent launch:
@@ -959,19 +966,31 @@ static void allow_big_lds(K kernel, bool& done)
     }
 }
 
+template <typename S, bool SENT, int ROWB>
+static void launch_fp_k(dim3 grid, size_t shm, const FpArgs* a, hipStream_t s)
+{
+    static bool done = false;
+    allow_big_lds(k_for_proj_diff<S, SENT, ROWB>, done);
+    hipLaunchKernelGGL((k_for_proj_diff<S, SENT, ROWB>), grid, dim3(FP_THREADS), shm, s, a);
+}
+
 template <typename S>
 static void launch_fp_t(const Plan& p, int subset, hipStream_t s)
 {
-    static bool done0 = false, done1 = false;
-    allow_big_lds(k_for_proj_diff<S, false>, done0);
-    allow_big_lds(k_for_proj_diff<S, true>, done1);
     const int n_list = p.subset_fp_off[subset + 1] - p.subset_fp_off[subset];
     const int group = fp_group(p.proj, n_list);
     dim3 grid((n_list + group - 1) / group, p.vol_y);
     const size_t shm = fp_lds_bytes(p.geo.fp_bz, p.vol_x);
     const FpArgs* a = p.d_fp_args + subset;
-    hipLaunchKernelGGL((k_for_proj_diff<S, false>), grid, dim3(FP_THREADS), shm, s, a);
-    hipLaunchKernelGGL((k_for_proj_diff<S, true>), grid, dim3(FP_THREADS), shm, s, a);
+    // the sentinel-free slices (nearly all) take a variant with the band
+    // pitch fixed at compile time for the common widths
+    switch (p.vol_x) {
+    case 512: launch_fp_k<S, false, 513 * 8>(grid, shm, a, s); break;
+    case 1024: launch_fp_k<S, false, 1025 * 8>(grid, shm, a, s); break;
+    case 2048: launch_fp_k<S, false, 2049 * 8>(grid, shm, a, s); break;
+    default: launch_fp_k<S, false, 0>(grid, shm, a, s); break;
+    }
+    launch_fp_k<S, true, 0>(grid, shm, a, s);
 }
 
 void launch_fp_diff(const Plan& p, int subset, hipStream_t s)

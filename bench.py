@@ -681,6 +681,15 @@ def main():
         rkernel = "k_wbp_bp_rowfft (2x-grid back-projection + row FFT)"
     avg_ms = kms / max(launches, 1)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    if sart and "fp_diff" in kernels:
+        # forward projection + difference, algorithmic bytes per launch: 4 B
+        # per voxel read + s_p per raw sample read and per difference written
+        # (one subset of tilts); VALU-bound, so the HBM fraction is low
+        fl, fms = plan.kernel_stats(L.K_FP)
+        fp_bytes = 4.0 * vox + 2.0 * s_p * (n / w["subsets"]) * vol_x * vol_y
+        fp_ms = fms / max(fl, 1)
+        kernels["fp_diff"].update({"avg_launch_ms": round(fp_ms, 4), "bytes_per_launch": fp_bytes,
+                                   "hbm_frac": round(fp_bytes / (fp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
     traffic = measured_traffic() if name == "sart1024" else None
     result = {
         "metric": METRIC,

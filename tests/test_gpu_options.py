@@ -191,3 +191,23 @@ def test_widest_supported_slices(gpu_available):
         err = rel_l2(gpu, ref)
         print("rel_l2", err)
         assert err <= 1e-5
+
+
+@pytest.mark.parametrize("px,storage", [(1024, L.STORAGE_F32), (2048, L.STORAGE_F16),
+                                        (1000, L.STORAGE_F32)])
+def test_forward_projector_widths(gpu_available, px, storage):
+    # VolX 512 / 1024 / 2048 take forward-projection variants whose band row
+    # pitch is a compile-time constant; other widths the general one
+    tilts = phantom.tilt_range(-60, 60, 4)
+    proj, _ = phantom.tilt_series(83, px, 2, tilts, 64, n_ellipsoids=60, axis_min=6,
+                                  axis_max=50, noise_frac=0.05)
+    print(f"case fp widths px={px} storage={storage}", flush=True)
+    plan = Plan(px, 2, tilts, 64, L.METHOD_SART, sart_iters=2, sart_subsets=3, relax=0.3,
+                subset_order=L.SUBSETS_INTERLEAVED, positivity=False, storage=storage)
+    gpu = plan.execute_host(proj)
+    plan.close()
+    ref = _ref(proj, tilts, L.METHOD_SART, 64, iters=2, n_subsets=3, relax=0.3, order=1,
+               positivity=0)
+    err = rel_l2(gpu, ref)
+    print("rel_l2", err)
+    assert err <= (TOL32 if storage == L.STORAGE_F32 else TOL16)
