@@ -52,7 +52,7 @@ SART_BENCH = dict(method="sart", relax=0.3, order="interleaved", positivity=Fals
 CONFIGS = {
     "sart1024": dict(SART_BENCH, px=1024, vol_z=256, step=2.0, n_ell=200, amax=40.0, axis=85.0,
                      shift=3.0, noise=0.10, iters=20, subsets=5, storage="f32", seed=20261203,
-                     cpu_slices=32),
+                     cpu_slices=128),
     "wbp512": dict(method="wbp", px=512, vol_z=128, step=3.0, n_ell=200, amax=40.0, axis=0.0,
                    shift=0.0, noise=0.05, storage="f32", seed=20261202, cpu_slices=512),
     "sart2048-f16": dict(SART_BENCH, px=2048, vol_z=512, step=2.0, n_ell=2000, amax=80.0,
