@@ -197,8 +197,10 @@ __global__ __launch_bounds__(256) void k_weight(const float* __restrict__ in, si
 // Workgroup = BP_TX x TZ voxels of one slice, lane = x, each thread walks
 // TZ z-rows (TZ = 16 by default, 8 a tuning choice).  For each chunk of CH
 // tilts (16 or 8) the needed part of every
-// sinogram row is staged in LDS as pairs (p[i], p[i+1]-p[i]) so that one
-// ds_read_b64 and one FMA give the linear interpolation.  Inclusion comes
+// sinogram row is staged in LDS as pairs (p[i] - i d[i], d[i]), d[i] =
+// p[i+1]-p[i], so that one ds_read_b64 and one FMA with the window
+// coordinate itself give the linear interpolation (slices with empty samples
+// stage (p[i], d[i]) and weight d by the fraction).  Inclusion comes
 // from the FP64 interval tables: tiles the host classified as fully inside
 // skip all per-lane tests and count per tile; partial tiles test x against
 // the (tilt, z) interval.  Sums and counts are FP32/int per voxel, in tilt
@@ -207,9 +209,15 @@ __global__ __launch_bounds__(256) void k_weight(const float* __restrict__ in, si
 // ------------------------------------------------------------------------
 // One tile: voxels x0 .. x0+BP_TX-1, z0 .. z0+TZ-1 of slice y.  Ends
 // after a barrier, with the LDS free for the next tile.
-template <typename S, int TZ, int CH, bool WBP = false>
-__device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_win(TZ)],
-                                        int xt, int zt, int y)
+//
+// SENT: the slice's sinogram rows may hold empty samples.  Without them the
+// staged pair of window index i is (q_i, d_i) with q_i = q(w0 + i) - i d_i,
+// so the interpolation at window coordinate u is the single FMA
+// q_i + u d_i (no fraction); with them the pair is (q(w0 + i), d_i) and the
+// value q + fract(u) d, whose sentinel test must see the exact -1e30 scale.
+template <typename S, int TZ, int CH, bool WBP, bool SENT>
+__device__ __forceinline__ void bp_tile_body(cptr<BpArgs> A, float2 (&pairs)[CH][bp_win(TZ)],
+                                             int xt, int zt, int y)
 {
     const gptr<const S> sino = (gptr<const S>)A->sino;
     const int rows_per_slice = A->rows_per_slice, stride = A->stride, proj = A->proj;
@@ -222,7 +230,6 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
     const float relax = A->relax;
     const int positivity = A->positivity;
     const float scale = A->scale;
-    const cptr<int> sflag = as_cptr(A->sflag);
     const gptr<float> vol = as_gptr(A->vol);
     const int vol_x = A->vol_x, vol_z = A->vol_z;
     int* vflag = A->vflag;
@@ -231,7 +238,6 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
     const int x0 = xt * BP_TX, z0 = zt * TZ;
     const int x = x0 + tid;
     const int xc = min(x, vol_x - 1);
-    const bool sent = sflag != nullptr && sflag[y] != 0;
     const gptr<float> vslice = vol + (size_t)y * vol_z * vol_x;
 
     // the voxels this thread updates, loaded now and consumed at the end
@@ -297,7 +303,9 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
                         if (r <= thr) r = kSentinel;
                     }
                     const float d = r - p;
-                    pairs[h + q][tid + m * BP_TX] = make_float2(fmaf((float)gv[q][m], d, p), d);
+                    // SENT: the extended value; else that minus i d
+                    const int gi = SENT ? gv[q][m] : gv[q][m] - (tid + m * BP_TX);
+                    pairs[h + q][tid + m * BP_TX] = make_float2(fmaf((float)gi, d, p), d);
                 }
             }
         }
@@ -313,15 +321,27 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
             const float cc = scale * sc.x, ss = scale * sc.y;
             const float u0 = fmaf(xf, cc, __int_as_float(tw.z));
             const float2* pj = pairs[j];
-            if (tw.y == 1 && !sent) {
+            if (tw.y == 1 && !SENT) {
                 full_cnt += 1;
 #pragma unroll
                 for (int k = 0; k < TZ; k++) {
                     float u = fmaf((float)k, ss, u0);
                     int i = (int)u;
-                    float f = __builtin_amdgcn_fractf(u);
                     float2 pr = pj[i];
-                    acc[k] += fmaf(f, pr.y, pr.x);
+                    acc[k] += fmaf(u, pr.y, pr.x);
+                }
+            } else if (!SENT) {
+                const cptr<int2> lh = lohi + (size_t)tr.tilt * vol_z;
+#pragma unroll
+                for (int k = 0; k < TZ; k++) {
+                    const int2 r = ldc(lh + min(z0 + k, vol_z - 1));
+                    float u = fmaf((float)k, ss, u0);
+                    int i = (int)u;
+                    float2 pr = pj[i];
+                    float v = fmaf(u, pr.y, pr.x);
+                    bool in = (x >= r.x) & (x <= r.y);
+                    acc[k] += in ? v : 0.0f;
+                    cnt[k] += in ? 1 : 0;
                 }
             } else {
                 const cptr<int2> lh = lohi + (size_t)tr.tilt * vol_z;
@@ -367,6 +387,17 @@ __device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_w
         }
     }
     if (wrote_sent) atomicOr(vflag + y, 1);
+}
+
+template <typename S, int TZ, int CH, bool WBP = false>
+__device__ __forceinline__ void bp_tile(cptr<BpArgs> A, float2 (&pairs)[CH][bp_win(TZ)],
+                                        int xt, int zt, int y)
+{
+    const cptr<int> sflag = as_cptr(A->sflag);
+    if (sflag != nullptr && sflag[y] != 0)
+        bp_tile_body<S, TZ, CH, WBP, true>(A, pairs, xt, zt, y);
+    else
+        bp_tile_body<S, TZ, CH, WBP, false>(A, pairs, xt, zt, y);
 }
 
 template <typename S, int TZ, int CH>
@@ -423,8 +454,9 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj_batch(const BpItem* __restr
 // Recon/GDiffProj.cu:7-21 fused).
 //
 // One workgroup (FP_THREADS) per (slice, group of tilts).  The slice is
-// streamed through LDS in z-bands of bz cell rows; band entry (z, x) holds the
-// interpolation pair (V[z][x], V[z][x+1]-V[z][x]), so a bilinear sample is two
+// streamed through LDS in z-bands of bz cell rows; band entry (z, e) holds the
+// interpolation pair (V[z][x] - e dV, dV), dV = V[z][x+1]-V[z][x], x = e - 1
+// (slices with empty voxels: (V[z][x], dV)), so a bilinear sample is two
 // ds_read_b64 (rows z and z+1) and three FMAs.  Entry 0 of a row is a guard for
 // x in [-1, 0) (linear extrapolation), which lets the entry index be a plain
 // truncation of x + 1.  The host assigns the group's (tilt, 64-ray tile)
@@ -445,18 +477,24 @@ struct Geo4 {
 // The two cells of a bilinear sample: g.x is the byte offset of the
 // sample's lower row in the band buffer, xs = x + 1 (the +1 skips the guard
 // pair), rowb the byte pitch of a band row.
+//
+// QF (slices without empty voxels): entry e holds (V_e - e dV_e, dV_e), the
+// value at xs is then (V_e - e dV_e) + xs dV_e and the slope's weight is xs
+// itself.  Otherwise entry e holds (V_e, dV_e) and the weight is fract(xs),
+// so that the sentinel scale of an empty voxel reaches the value unchanged.
 struct FpCells {
-    float2 e0, e1;  // (V, dV/dx) of the lower and upper row
-    float f;        // x fraction
+    float2 e0, e1;  // entries of the lower and upper row
+    float f;        // weight of the slopes: xs (QF) or its fraction
 };
 
+template <bool QF>
 __device__ __forceinline__ FpCells fp_fetch(const char* __restrict__ band, float4 g, float xs,
                                             int rowb)
 {
     const int ix = (int)xs;
     const char* p = band + __float_as_int(g.x) + 8 * ix;
     FpCells c;
-    c.f = __builtin_amdgcn_fractf(xs);
+    c.f = QF ? xs : __builtin_amdgcn_fractf(xs);
     c.e0 = *(const float2*)p;
     c.e1 = *(const float2*)(p + rowb);
     return c;
@@ -502,7 +540,7 @@ __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb_r
         for (int q = 0; q < 4; q++) {
             float xs = tf1 - g4.g[q].z;  // x + 1
             if (MASKED) xs = __builtin_amdgcn_fmed3f(xs, 1.0f, xmax1);
-            c[q] = fp_fetch(band, g4.g[q], xs, rowb);
+            c[q] = fp_fetch<!SENT>(band, g4.g[q], xs, rowb);
         }
         // keep the reads of the group ahead of their uses: scheduled next to
         // each use, every sample would wait out a full LDS latency
@@ -526,7 +564,7 @@ __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb_r
         const float4 g = ldc(G + m);
         float xs = tf1 - g.z;
         if (MASKED) xs = __builtin_amdgcn_fmed3f(xs, 1.0f, xmax1);
-        const FpCells c = fp_fetch(band, g, xs, rowb);
+        const FpCells c = fp_fetch<!SENT>(band, g, xs, rowb);
         if (MASKED) {
             const float v = fp_value(c, g);
             bool in = (unsigned)(k0 + m - rr.x) < (unsigned)rr.y;
@@ -617,8 +655,10 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
                 for (int r = 0; r < FP_STAGE_ROWS; r++) {
                     if (h + r < rows) {
                         const float dx = vb[r] - va[r];
+                        const float v0 = xi < 0 ? va[r] - dx : va[r];
+                        // SENT: (V, dV); else (V - e dV, dV) for entry e = col
                         band[(h + r) * pitch + col] =
-                            xi < 0 ? make_float2(va[r] - dx, dx) : make_float2(va[r], dx);
+                            make_float2(SENT ? v0 : fmaf(-(float)col, dx, v0), dx);
                     }
                 }
             }
