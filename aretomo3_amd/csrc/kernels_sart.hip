@@ -7,6 +7,7 @@
 //                                    the sinogram of slice y is contiguous
 //   d_diff    [VolY][max_subset][stride]  SART residual of the current subset
 //   d_vol     [VolY][VolZ][VolX]     SART state, x fastest (XZY, unflipped)
+#include <atomic>
 #include <climits>
 
 #include "atr_internal.h"
@@ -995,21 +996,25 @@ size_t fp_lds_bytes(int bz, int vol_x)
     return (size_t)(bz + 1) * (vol_x + 1) * sizeof(float2);
 }
 
-// Kernels with more than 64 KiB of dynamic LDS must say so once.
+// Kernels with more than 64 KiB of dynamic LDS must say so once per device;
+// `done` holds one bit per device the attribute was set on (devices 64 and
+// up set it at every launch).
 template <typename K>
-static void allow_big_lds(K kernel, bool& done)
+static void allow_big_lds(K kernel, std::atomic<unsigned long long>& done)
 {
-    if (!done) {
-        (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  FP_LDS_BUDGET);
-        done = true;
-    }
+    int dev = -1;
+    const bool known = hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64;
+    const unsigned long long bit = known ? 1ull << dev : 0ull;
+    if (known && (done.load(std::memory_order_relaxed) & bit)) return;
+    if (hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            FP_LDS_BUDGET) == hipSuccess && known)
+        done.fetch_or(bit, std::memory_order_relaxed);
 }
 
 template <typename S, bool SENT, int ROWB>
 static void launch_fp_k(dim3 grid, size_t shm, const FpArgs* a, hipStream_t s)
 {
-    static bool done = false;
+    static std::atomic<unsigned long long> done{0};
     allow_big_lds(k_for_proj_diff<S, SENT, ROWB>, done);
     hipLaunchKernelGGL((k_for_proj_diff<S, SENT, ROWB>), grid, dim3(FP_THREADS), shm, s, a);
 }
@@ -1043,7 +1048,7 @@ void launch_fp_diff(const Plan& p, int subset, hipStream_t s)
 template <typename S>
 static void fp_batch_attrs()
 {
-    static bool done0 = false, done1 = false;
+    static std::atomic<unsigned long long> done0{0}, done1{0};
     allow_big_lds(k_for_proj_diff_batch<S, false>, done0);
     allow_big_lds(k_for_proj_diff_batch<S, true>, done1);
 }
