@@ -690,6 +690,17 @@ def main():
         fp_ms = fms / max(fl, 1)
         kernels["fp_diff"].update({"avg_launch_ms": round(fp_ms, 4), "bytes_per_launch": fp_bytes,
                                    "hbm_frac": round(fp_bytes / (fp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
+        # flop model of both kernels against the FP32 vector peak: 8 flop per
+        # back-projection update, 14 per forward ray sample (L = VolZ/cos + 1.5
+        # samples per ray), averaged per launch
+        cos_t = np.cos(np.float32(0.017453) * tilts.astype(np.float32)).astype(np.float64)
+        samples = w["iters"] * vol_y * vol_x * float(np.sum((vol_z / cos_t + 1.5).astype(np.int64)))
+        fp_flop = 14.0 * samples / (w["iters"] * w["subsets"])
+        bp_flop = 8.0 * (1 + w["iters"]) * vox * n / (1 + w["iters"] * w["subsets"])
+        for kname, flop, ms in (("fp_diff", fp_flop, fp_ms), ("bp", bp_flop, avg_ms)):
+            tf = flop / (ms * 1e-3) / 1e12
+            kernels[kname]["flop_model"] = {"flop_per_launch": flop, "tflops": round(tf, 2),
+                                            "frac_of_fp32_vector_peak_157.3": round(tf / 157.3, 4)}
     traffic = measured_traffic() if name == "sart1024" else None
     result = {
         "metric": METRIC,
