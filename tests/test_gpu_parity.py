@@ -99,3 +99,38 @@ def test_golden_fixture(gpu_available, case):
     print("rel_l2", err)
     assert gpu.shape == g[case].shape
     assert err <= TOL32
+
+
+@pytest.mark.parametrize("method", [L.METHOD_SART, L.METHOD_WBP])
+def test_acquisition_order_and_asymmetric_tilts(gpu_available, method):
+    # tilts in a dose-symmetric acquisition order (0, +3, -3, +6, ...) over an
+    # asymmetric range, not sorted by angle: the subsets, tables and the WBP
+    # tilt list follow the given order exactly as the oracle does
+    from aretomo3_amd import phantom
+    order = [0.0]
+    for k in range(1, 16):
+        order += [3.0 * k, -3.0 * k]
+    tilts = np.array([t for t in order if -45.0 <= t <= 57.0], np.float32)
+    _, proj, _ = small_series(px=72, py=5, vol_z=28, tilts=tilts)
+    print(f"case acquisition order method={method} n={len(tilts)}", flush=True)
+    plan = Plan(72, 5, tilts, 28, method, sart_iters=3, sart_subsets=4)
+    gpu = plan.execute_host(proj)
+    ref = _oracle_series(proj, tilts, method, 28, iters=3, n_subsets=4)
+    err = rel_l2(gpu, ref)
+    print("rel_l2", err)
+    assert err <= TOL32
+
+
+@pytest.mark.parametrize("n_tilts,subsets", [(2, 1), (3, 3)])
+def test_sart_few_tilts(gpu_available, n_tilts, subsets):
+    # the smallest series: one or two tilts per subset, every subset a single
+    # forward-projection workgroup per slice
+    tilts = np.linspace(-30.0, 30.0, n_tilts).astype(np.float32)
+    _, proj, _ = small_series(px=40, py=3, vol_z=12, tilts=tilts)
+    print(f"case few tilts n={n_tilts} subsets={subsets}", flush=True)
+    plan = Plan(40, 3, tilts, 12, L.METHOD_SART, sart_iters=4, sart_subsets=subsets)
+    gpu = plan.execute_host(proj)
+    ref = _oracle_series(proj, tilts, 1, 12, iters=4, n_subsets=subsets)
+    err = rel_l2(gpu, ref)
+    print("rel_l2", err)
+    assert err <= TOL32
