@@ -211,3 +211,24 @@ def test_forward_projector_widths(gpu_available, px, storage):
     err = rel_l2(gpu, ref)
     print("rel_l2", err)
     assert err <= (TOL32 if storage == L.STORAGE_F32 else TOL16)
+
+
+@pytest.mark.parametrize("storage", [L.STORAGE_F16, L.STORAGE_BF16])
+def test_unfilled_sentinels_narrow_storage(gpu_available, storage):
+    # empty samples held in FP16/BF16 (which cannot hold -1e30) are widened
+    # back to the sentinel when staged: same empty-sample rules as FP32
+    tilts, raw, shifts = small_series(px=48, py=40, vol_z=16, axis=85.0, shift=3.0)
+    axes = np.full(len(tilts), 85.0, np.float32)
+    print(f"case sentinels storage={storage}", flush=True)
+    aligned = oracle.prestage(raw, axes, shifts, 0, 0)
+    plan = Plan(48, 40, tilts, 16, L.METHOD_SART, tilt_axis=axes, shifts=shifts, random_fill=False,
+                sart_iters=2, sart_subsets=3, fill_seed=5, storage=storage)
+    gpu = plan.execute_host(raw)
+    ref = oracle.recon_slices(aligned, tilts, 1, 16, list(range(aligned.shape[1])),
+                              iters=2, n_subsets=3, seed=5)
+    assert np.isfinite(gpu).all()
+    empty_gpu, empty_ref = gpu < -1e10, ref < -1e10
+    assert np.array_equal(empty_gpu, empty_ref)
+    err = rel_l2(gpu[~empty_ref], ref[~empty_ref])
+    print("rel_l2", err)
+    assert err <= TOL16
