@@ -297,8 +297,9 @@ __device__ __forceinline__ void bp_tile_body(cptr<BpArgs> A, float2 (&pairs)[CH]
                 for (int m = 0; m < 2; m++) {
                     if (m == 1 && tid >= WIN - BP_TX) break;
                     float p = pv[q][m], r = rv[q][m];
-                    if constexpr (!std::is_same<S, float>::value) {
-                        // narrow storage cannot hold -1e30: widen its empty marks
+                    if constexpr (SENT && !std::is_same<S, float>::value) {
+                        // narrow storage cannot hold -1e30: widen its empty
+                        // marks (a slice without them has nothing to widen)
                         const float thr = empty_threshold<S>();
                         if (p <= thr) p = kSentinel;
                         if (r <= thr) r = kSentinel;
