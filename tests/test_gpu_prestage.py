@@ -108,3 +108,22 @@ def test_wbp_fft_prime_factors(gpu_available, px, py, vol_z):
     err = rel_l2(gpu, ref)
     print("rel_l2", err)
     assert err <= TOL32
+
+
+def test_per_tilt_axes_and_many_patches(gpu_available):
+    # a tilt axis refined per tilt (84..86 deg; the aligned size follows the
+    # first tilt's axis) and a 6 x 6 patch grid, as -Patch 6 6 leaves it
+    tilts, raw, shifts = small_series(seed=41, px=72, py=56, vol_z=20, axis=85.0, shift=2.5)
+    n = len(tilts)
+    axes = np.linspace(84.0, 86.0, n).astype(np.float32)
+    lp = patch_table(9, n, 72, 56, 36)
+    print("case per-tilt axes, 36 patches", flush=True)
+    plan = Plan(72, 56, tilts, 20, L.METHOD_SART, tilt_axis=axes, shifts=shifts, local_params=lp,
+                sart_iters=2, sart_subsets=3, fill_seed=3)
+    gpu = plan.execute_host(raw)
+    ref = oracle_recon(raw, tilts, L.METHOD_SART, 20, axes, shifts, lp, seed=3, iters=2,
+                       n_subsets=3)
+    assert gpu.shape == ref.shape
+    err = rel_l2(gpu, ref)
+    print("rel_l2", err)
+    assert err <= TOL32
