@@ -232,3 +232,25 @@ def test_unfilled_sentinels_narrow_storage(gpu_available, storage):
     err = rel_l2(gpu[~empty_ref], ref[~empty_ref])
     print("rel_l2", err)
     assert err <= TOL16
+
+
+@pytest.mark.parametrize("method", [L.METHOD_SART, L.METHOD_WBP])
+def test_strided_input_equals_dense(gpu_available, method):
+    # raw frames inside a larger buffer (row pitch ProjX + 7, frame pitch
+    # (ProjY + 3) rows), with the pre-stage: the same volume, bit for bit, as
+    # the dense input
+    tilts, raw, shifts = small_series(px=56, py=30, vol_z=18, axis=85.0, shift=2.0)
+    n = len(tilts)
+    axes = np.full(n, 85.0, np.float32)
+    rs, fr = 56 + 7, 30 + 3
+    big = np.full((n, fr, rs), 7.0, np.float32)
+    big[:, :30, :56] = raw
+    print(f"case strided input method={method}", flush=True)
+    kw = dict(tilt_axis=axes, shifts=shifts, sart_iters=2, sart_subsets=2, fill_seed=9)
+    dense = Plan(56, 30, tilts, 18, method, **kw)
+    want = dense.execute_host(raw)
+    dense.close()
+    strided = Plan(56, 30, tilts, 18, method, tilt_stride=fr * rs, row_stride=rs, **kw)
+    got = strided.execute_host(big)
+    strided.close()
+    np.testing.assert_array_equal(got, want)
