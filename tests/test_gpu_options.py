@@ -254,3 +254,40 @@ def test_strided_input_equals_dense(gpu_available, method):
     got = strided.execute_host(big)
     strided.close()
     np.testing.assert_array_equal(got, want)
+
+
+def test_concurrent_plans_on_user_streams(gpu_available):
+    # two plans of different series, each run twice on its own caller-created
+    # stream with the launches interleaved: a plan owns all its scratch, so
+    # each result equals the plan's serial result bit for bit
+    import ctypes
+    from aretomo3_amd import DeviceArray, synchronize
+    series = [small_series(seed=61, px=64, py=10, vol_z=24),
+              small_series(seed=62, px=48, py=14, vol_z=20,
+                           tilts=np.arange(-54.0, 55.0, 6.0).astype(np.float32))]
+    plans, inputs, outputs, wants, streams = [], [], [], [], []
+    print("case concurrent plans on user streams", flush=True)
+    for k, (tilts, proj, _) in enumerate(series):
+        px, py, vz = proj.shape[2], proj.shape[1], (24, 20)[k]
+        p = Plan(px, py, tilts, vz, L.METHOD_SART, sart_iters=3, sart_subsets=3)
+        wants.append(p.execute_host(proj))
+        plans.append(p)
+        inputs.append(DeviceArray.from_numpy(proj))
+        outputs.append([DeviceArray(4 * int(np.prod(p.volume_shape))) for _ in range(2)])
+        s = ctypes.c_void_p()
+        L.check(L.lib().atr_stream_create(ctypes.byref(s)))
+        streams.append(s.value)
+    for rep in range(2):
+        for k in range(2):
+            plans[k].execute(inputs[k], outputs[k][rep], False, streams[k])
+    for s in streams:
+        L.check(L.lib().atr_stream_synchronize(s))
+    synchronize()
+    for k in range(2):
+        for rep in range(2):
+            got = outputs[k][rep].to_numpy(plans[k].volume_shape)
+            np.testing.assert_array_equal(got, wants[k])
+    for s in streams:
+        L.check(L.lib().atr_stream_destroy(s))
+    for p in plans:
+        p.close()
