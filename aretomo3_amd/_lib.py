@@ -36,7 +36,7 @@ EXPORTS = [
     "atr_batch_reset_stats", "atr_batch_destroy",
     "atr_server_config_init", "atr_server_create", "atr_server_submit", "atr_server_wait",
     "atr_server_stats", "atr_server_destroy", "atr_host_alloc", "atr_host_free",
-    "atr_plan_autotune", "atr_plan_set_tile_shape",
+    "atr_plan_autotune", "atr_plan_set_tile_shape", "atr_fp_band_candidates",
     "atr_aligned_size", "atr_mrc_read_info", "atr_mrc_read_sections", "atr_mrc_write", "atr_volume_suffix",
     "atr_aln_read", "atr_aln_get_info", "atr_aln_get_global", "atr_aln_get_local",
     "atr_aln_get_dark", "atr_aln_destroy", "atr_aln_write", "atr_tilt_file_read",
@@ -182,6 +182,7 @@ def lib():
         "atr_host_free": (i, [vp]),
         "atr_plan_autotune": (i, [vp, vp, pi, pi, pi]),
         "atr_plan_set_tile_shape": (i, [vp, i, i, i]),
+        "atr_fp_band_candidates": (i, [i, i, pi, i, pi]),
         "atr_aligned_size": (i, [i, i, f, pi, pi]),
         "atr_mrc_read_info": (i, [ctypes.c_char_p, ctypes.POINTER(MrcInfo)]),
         "atr_mrc_read_sections": (i, [ctypes.c_char_p, i, i, vp]),

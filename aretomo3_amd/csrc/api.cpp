@@ -192,8 +192,9 @@ static atr_status plan_sizes(const atr_series_desc* d, Plan& p)
             !make_fft(p.vol_z, f) || !make_fft(p.vol_x, f))
             return fail(ATR_ERR_UNSUPPORTED_SIZE,
                         "WBP: ProjX, VolX or VolZ has no FFT plan");
-        if (p.proj > 4096 || p.vol_x > 2048 || p.vol_z > 2048)
-            return fail(ATR_ERR_UNSUPPORTED_SIZE, "WBP supports ProjX <= 4096, VolZ <= 2048");
+        if (p.vol_x > 2048 || p.vol_z > 2048)
+            return fail(ATR_ERR_UNSUPPORTED_SIZE,
+                        "WBP supports ProjX (= VolX) <= 2048 and VolZ <= 2048");
         // slab of slices whose spectra are buffered between the WBP kernels
         size_t per_slice = (size_t)3 * p.vol_z * (p.vol_x / 2 + 1) * sizeof(float2) +
                            (size_t)4 * p.vol_z * p.vol_x * sizeof(float);
@@ -623,6 +624,9 @@ static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_
         if (bz_out) *bz_out = 0;
         return ATR_OK;
     }
+    if (p.batch_refs > 0)
+        return fail(ATR_ERR_INVALID_ARG,
+                    "autotune: the plan belongs to a batch (destroy the batch first)");
     const TuneKey key{p.vol_x, p.vol_z, p.proj, p.n, p.n_subsets, p.storage};
     TuneChoice best{p.bp_tz, p.bp_chunk, p.geo.fp_bz};
     bool cached = false;
@@ -643,30 +647,43 @@ static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_
         ATR_HIP(hipMemsetAsync(p.d_vol, 0, sizeof(float) * p.vol_x * p.vol_y * p.vol_z, s));
         ATR_HIP(hipMemsetAsync(p.d_sflag, 0, sizeof(int) * p.vol_y, s));
         ATR_HIP(hipMemsetAsync(p.d_vflag, 0, sizeof(int) * p.vol_y, s));
-        const int bzmax = fp_band_rows(p.vol_x);
-        const int bz0 = fp_balanced_rows(bzmax, p.vol_z);
+        ATR_HIP(hipStreamSynchronize(s));
+        (void)hipGetLastError();
+        std::vector<int> bzs;
+        fp_band_candidates(p.vol_x, p.vol_z, bzs);
         const bool has_bp = p.subset_off[1] > p.subset_off[0];
+        // A candidate whose launch fails (an error, not a time) is dropped: a
+        // failed launch would otherwise time as ~0 ms and win.
         float best_bp = 1e30f;
+        bool bp_ok = false;
         const int shapes[4][2] = {{16, 16}, {16, 8}, {8, 16}, {8, 8}};
         for (auto& sh : shapes) {
-            if ((st = sart_tables(p, sh[0], sh[1], bz0)) != ATR_OK) return st;
+            if ((st = sart_tables(p, sh[0], sh[1], bzs[0])) != ATR_OK) return st;
             const float ms = time_ms(s, 3, [&] { launch_bp(p, has_bp ? 1 : 0, s); });
+            if (hipGetLastError() != hipSuccess) continue;
             if (ms < best_bp) {
                 best_bp = ms;
                 best.tz = sh[0];
                 best.chunk = sh[1];
+                bp_ok = true;
             }
         }
         float best_fp = 1e30f;
-        std::vector<int> bzs = {bz0, fp_balanced_rows(std::max(2, bzmax * 2 / 3), p.vol_z),
-                                fp_balanced_rows(std::max(2, bzmax / 2), p.vol_z)};
+        bool fp_ok = false;
         for (int bz : bzs) {
             if ((st = sart_tables(p, best.tz, best.chunk, bz)) != ATR_OK) return st;
             const float ms = time_ms(s, 3, [&] { launch_fp_diff(p, 0, s); });
+            if (hipGetLastError() != hipSuccess) continue;
             if (ms < best_fp) {
                 best_fp = ms;
                 best.bz = bz;
+                fp_ok = true;
             }
+        }
+        if (!bp_ok || !fp_ok) {
+            // nothing launched: restore the default shape and report it
+            (void)sart_tables(p, BP_TZ, BP_CHUNK, bzs[0]);
+            return fail(ATR_ERR_HIP, "autotune: no candidate tile shape launched");
         }
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = best;
@@ -804,9 +821,26 @@ atr_status atr_plan_set_tile_shape(atr_plan* plan, int bp_tz, int bp_chunk, int 
     if (p.method != ATR_METHOD_SART) return fail(ATR_ERR_INVALID_ARG, "tile shapes are SART settings");
     if ((bp_tz != 8 && bp_tz != 16) || (bp_chunk != 8 && bp_chunk != 16))
         return fail(ATR_ERR_INVALID_ARG, "bp_tz and bp_chunk must be 8 or 16");
-    if (fp_rows < 2 || fp_rows > fp_band_rows(p.vol_x))
-        return fail(ATR_ERR_INVALID_ARG, "fp band rows outside [2, LDS limit]");
+    if (fp_rows < 1 || fp_rows > fp_band_rows(p.vol_x))
+        return fail(ATR_ERR_INVALID_ARG,
+                    "fp band rows outside [1, " + std::to_string(fp_band_rows(p.vol_x)) +
+                        "] (the LDS limit at VolX " + std::to_string(p.vol_x) + ")");
+    if (p.batch_refs > 0)
+        return fail(ATR_ERR_INVALID_ARG,
+                    "set_tile_shape: the plan belongs to a batch (destroy the batch first)");
     return sart_tables(p, bp_tz, bp_chunk, fp_rows);
+}
+
+atr_status atr_fp_band_candidates(int vol_x, int vol_z, int* rows, int cap, int* n)
+{
+    if (n == nullptr || vol_x < 2 || vol_z < 2)
+        return fail(ATR_ERR_INVALID_ARG, "atr_fp_band_candidates: bad size or NULL count");
+    std::vector<int> c;
+    fp_band_candidates(vol_x, vol_z, c);
+    *n = (int)c.size();
+    if (rows)
+        for (int i = 0; i < std::min(cap, (int)c.size()); i++) rows[i] = c[i];
+    return ATR_OK;
 }
 
 atr_status atr_plan_autotune(atr_plan* plan, void* stream, int* bp_tz, int* bp_chunk,

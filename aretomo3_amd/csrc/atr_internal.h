@@ -144,6 +144,9 @@ struct Plan {
     int* d_vflag = nullptr;           // [vol_y] volume slice holds sentinels
     double* d_vstats = nullptr;       // [vol_y][2] fill statistics of volume slices
     int bp_tz = BP_TZ, bp_chunk = BP_CHUNK;   // back-projection tile shape (autotuned)
+    // batches holding this plan: their item arrays and captured graphs point
+    // at its tables, so the tables must not be rebuilt while this is > 0
+    int batch_refs = 0;
     size_t tab_bytes[9] = {0};        // sizes of the tile-shape dependent tables
     BpArgs* d_bp_args = nullptr;      // SART step arguments: initial BP, then per subset
     FpArgs* d_fp_args = nullptr;      // per subset
@@ -202,10 +205,14 @@ struct FpArgs {
     int diff_rows;
 };
 // Entries of the batched (work-queue) launches; item0 = first queue index.
+// shape: the series' back-projection tile shape, bp_shape(bp_tz, bp_chunk).
 struct BpItem {
     BpArgs a;
     int item0, n_items_y;      // items of one slice = ztiles * xtiles
+    int shape;
 };
+// 0: 16 z rows x 16 tilts per stage, 1: 16 x 8, 2: 8 x 16, 3: 8 x 8
+inline int bp_shape(int tz, int chunk) { return (tz == 8 ? 2 : 0) + (chunk == 8 ? 1 : 0); }
 struct FpItem {
     FpArgs a;
     int item0, n_groups;       // items of one slice = tilt groups
@@ -266,6 +273,7 @@ bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, 
                      int bz, Geometry& g);
 int fp_band_rows(int vol_x);
 int fp_balanced_rows(int bz_max, int vol_z);
+void fp_band_candidates(int vol_x, int vol_z, std::vector<int>& rows);
 int fp_group(int proj, int n_list);
 void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, int group,
                       std::vector<int>& tab);

@@ -22,7 +22,7 @@ struct atr_batch {
         int first;      // first entry in the item array of its kind
         int n_entries;
         int total;      // queue items
-        int counter;    // counter slot (FP steps use two: SENT false / true)
+        int counter;    // queue counter slot
     };
     std::vector<Step> steps;
     atr::BpItem* d_bp = nullptr;
@@ -76,6 +76,7 @@ static atr_status batch_build(atr_batch* b)
             e.a = bp_args(p, p.d_sino, p.n, p.d_lists, n_all, 1.0f, true);
             e.item0 = st.total;
             e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
+            e.shape = bp_shape(p.bp_tz, p.bp_chunk);
             st.total += p.vol_y * e.n_items_y;
             bp.push_back(e);
             st.n_entries++;
@@ -84,8 +85,7 @@ static atr_status batch_build(atr_batch* b)
     }
     for (int it = 0; it < max_iters; it++)
         for (int sb = 0; sb < max_sub; sb++) {
-            atr_batch::Step sf{true, (int)fp.size(), 0, 0, b->n_counters};
-            b->n_counters += 2;
+            atr_batch::Step sf{true, (int)fp.size(), 0, 0, b->n_counters++};
             atr_batch::Step sp{false, (int)bp.size(), 0, 0, b->n_counters++};
             for (atr_plan* h : b->plans) {
                 const Plan& p = h->p;
@@ -106,6 +106,7 @@ static atr_status batch_build(atr_batch* b)
                                   p.relax, false);
                     e.item0 = sp.total;
                     e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
+                    e.shape = bp_shape(p.bp_tz, p.bp_chunk);
                     sp.total += p.vol_y * e.n_items_y;
                     bp.push_back(e);
                     sp.n_entries++;
@@ -224,9 +225,6 @@ atr_status atr_batch_create(atr_plan* const* plans, int n_plans, int flags, atr_
             return fail(ATR_ERR_INVALID_ARG, "atr_batch_create: batches hold SART plans only");
         if (plans[i]->p.storage != plans[0]->p.storage)
             return fail(ATR_ERR_INVALID_ARG, "atr_batch_create: plans differ in storage type");
-        if (plans[i]->p.bp_tz != BP_TZ || plans[i]->p.bp_chunk != BP_CHUNK)
-            return fail(ATR_ERR_INVALID_ARG,
-                        "atr_batch_create: batches use the default back-projection tile shape");
         for (int j = 0; j < i; j++)
             if (plans[j] == plans[i])
                 return fail(ATR_ERR_INVALID_ARG, "atr_batch_create: a plan appears twice");
@@ -241,6 +239,9 @@ atr_status atr_batch_create(atr_plan* const* plans, int n_plans, int flags, atr_
         delete b;
         return st;
     }
+    // the item arrays now hold the plans' table pointers: retuning a member
+    // plan (which rebuilds them) is refused until the batch is destroyed
+    for (atr_plan* h : b->plans) h->p.batch_refs += 1;
     *batch = b;
     return ATR_OK;
 }
@@ -283,6 +284,7 @@ void atr_batch_destroy(atr_batch* batch)
 {
     if (!batch) return;
     (void)hipStreamSynchronize(batch->stream);
+    for (atr_plan* h : batch->plans) h->p.batch_refs -= 1;
     batch_free(batch);
     delete batch;
 }

@@ -366,6 +366,20 @@ int fp_balanced_rows(int bz_max, int vol_z)
     return (cells + nb - 1) / nb;
 }
 
+// Band heights the autotuner times: the largest that fits the LDS, then
+// about 2/3 and 1/2 of it, each balanced over the slice's cell rows, none
+// above the LDS limit, no height twice.  At VolX >= 6400 only one row fits
+// (rows of VolX + 1 pairs), and the list is {1}.
+void fp_band_candidates(int vol_x, int vol_z, std::vector<int>& rows)
+{
+    const int bzmax = fp_band_rows(vol_x);
+    rows.clear();
+    for (int want : {bzmax, bzmax * 2 / 3, bzmax / 2}) {
+        const int bz = std::min(bzmax, fp_balanced_rows(std::max(1, want), vol_z));
+        if (std::find(rows.begin(), rows.end(), bz) == rows.end()) rows.push_back(bz);
+    }
+}
+
 // (tilt-in-group, tile) of every pair index p = wave + 16 i of the forward
 // projector: j << 16 | tile.
 // Tilts per forward-projection workgroup for a list of n_list tilts: at most

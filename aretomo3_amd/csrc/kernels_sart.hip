@@ -419,35 +419,68 @@ __global__ __launch_bounds__(BP_TX) void k_wbp_grid_bp(const BpArgs* __restrict_
     bp_tile<S, BP_TZ, BP_CHUNK, true>(as_cptr(a), pairs, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-// Block-wide work queue: thread 0 takes the next item index, every thread
-// sees it.  Items past the end tell every block to leave.
-__device__ __forceinline__ int next_item(int* __restrict__ counter)
+// Block-wide work queue of the persistent launches.  Thread 0 claims item
+// indices with an atomic on the queue counter; every thread sees the value
+// through LDS.  The item after the current one is claimed when the current
+// one starts, so the atomic's round trip overlaps the item's own loads
+// instead of stalling the block between items.  Indices are handed out in
+// increasing order, each to exactly one block, and a block leaves when it
+// holds an index past the end (every block therefore exits).
+__device__ __forceinline__ int publish_item(int v)
 {
     __shared__ int s_item;
-    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    if (threadIdx.x == 0) s_item = v;
     __syncthreads();
     const int it = uniform(s_item);
     __syncthreads();
     return it;
 }
 
+__device__ __forceinline__ int claim_item(int* __restrict__ counter)
+{
+    return threadIdx.x == 0 ? atomicAdd(counter, 1) : 0;
+}
+
+// Entry of queue index `it`.  Entries are ordered by their first item and a
+// block's indices only grow, so the search resumes where the last one ended.
+template <typename Item>
+__device__ __forceinline__ int entry_of(cptr<Item> E, int n_e, int it, int s)
+{
+    while (s + 1 < n_e && E[s + 1].item0 <= it) s++;
+    return s;
+}
+
+template <int TZ, int CH>
+__device__ __forceinline__ float2 (&bp_pairs(float2* lds))[CH][bp_win(TZ)]
+{
+    return *reinterpret_cast<float2(*)[CH][bp_win(TZ)]>(lds);
+}
+
 // Back-projection of many series in one persistent launch: blocks take
 // (series, slice, z-tile, x-tile) items from a queue; entries are ordered by
-// their first item.
+// their first item.  Each entry runs its own series' tile shape.
 template <typename S>
 __global__ __launch_bounds__(BP_TX) void k_back_proj_batch(const BpItem* __restrict__ e, int n_e,
                                                            int total, int* __restrict__ counter)
 {
-    __shared__ float2 pairs[BP_CHUNK][BP_WIN];
+    __shared__ float2 lds[BP_CHUNK * bp_win(BP_TZ)];   // the largest shape's stage
     const cptr<BpItem> E = as_cptr(e);
-    for (int it = next_item(counter); it < total; it = next_item(counter)) {
-        int s = 0;
-        while (s + 1 < n_e && E[s + 1].item0 <= it) s++;
+    int s = 0;
+    for (int it = publish_item(claim_item(counter)); it < total;) {
+        const int ahead = claim_item(counter);
+        s = entry_of(E, n_e, it, s);
         const int local = it - E[s].item0;
         const int per_y = E[s].n_items_y;
         const int y = local / per_y, r = local - y * per_y;
         const int zt = r / E[s].a.n_xtiles, xt = r - zt * E[s].a.n_xtiles;
-        bp_tile<S, BP_TZ, BP_CHUNK>(&E[s].a, pairs, xt, zt, y);
+        const cptr<BpArgs> A = &E[s].a;
+        switch (E[s].shape) {
+        case 1: bp_tile<S, 16, 8>(A, bp_pairs<16, 8>(lds), xt, zt, y); break;
+        case 2: bp_tile<S, 8, 16>(A, bp_pairs<8, 16>(lds), xt, zt, y); break;
+        case 3: bp_tile<S, 8, 8>(A, bp_pairs<8, 8>(lds), xt, zt, y); break;
+        default: bp_tile<S, 16, 16>(A, bp_pairs<16, 16>(lds), xt, zt, y); break;
+        }
+        it = publish_item(ahead);
     }
 }
 
@@ -722,21 +755,36 @@ __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(const FpArgs* __re
 }
 
 // Forward projection + difference of many series in one persistent launch:
-// blocks take (series, slice, tilt group) items from a queue.
-template <typename S, bool SENT>
+// blocks take (series, slice, tilt group) items from a queue.  A slice with
+// empty voxels runs the sentinel variant, the others the variant with the
+// band pitch fixed at compile time for VolX 512 / 1024 / 2048 (as the
+// per-series launch does), so one launch covers every slice.
+template <typename S>
 __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff_batch(const FpItem* __restrict__ e,
                                                                    int n_e, int total,
                                                                    int* __restrict__ counter)
 {
     extern __shared__ float2 band[];
     const cptr<FpItem> E = as_cptr(e);
-    for (int it = next_item(counter); it < total; it = next_item(counter)) {
-        int s = 0;
-        while (s + 1 < n_e && E[s + 1].item0 <= it) s++;
+    int s = 0;
+    for (int it = publish_item(claim_item(counter)); it < total;) {
+        const int ahead = claim_item(counter);
+        s = entry_of(E, n_e, it, s);
         const int local = it - E[s].item0;
         const int ng = E[s].n_groups;
         const int y = local / ng, gx = local - y * ng;
-        fp_block<S, SENT>(&E[s].a, band, gx, y);
+        const cptr<FpArgs> A = &E[s].a;
+        if (as_cptr(A->vflag)[y] != 0) {
+            fp_block<S, true>(A, band, gx, y);
+        } else {
+            switch (A->vol_x) {
+            case 512: fp_block<S, false, 513 * 8>(A, band, gx, y); break;
+            case 1024: fp_block<S, false, 1025 * 8>(A, band, gx, y); break;
+            case 2048: fp_block<S, false, 2049 * 8>(A, band, gx, y); break;
+            default: fp_block<S, false, 0>(A, band, gx, y); break;
+            }
+        }
+        it = publish_item(ahead);
     }
 }
 
@@ -1049,9 +1097,8 @@ void launch_fp_diff(const Plan& p, int subset, hipStream_t s)
 template <typename S>
 static void fp_batch_attrs()
 {
-    static std::atomic<unsigned long long> done0{0}, done1{0};
-    allow_big_lds(k_for_proj_diff_batch<S, false>, done0);
-    allow_big_lds(k_for_proj_diff_batch<S, true>, done1);
+    static std::atomic<unsigned long long> done{0};
+    allow_big_lds(k_for_proj_diff_batch<S>, done);
 }
 
 template <typename S>
@@ -1059,10 +1106,8 @@ static void launch_fp_batch_t(const FpItem* items, int n_items, int total, int* 
                               size_t shm, int grid, hipStream_t s)
 {
     fp_batch_attrs<S>();
-    hipLaunchKernelGGL((k_for_proj_diff_batch<S, false>), dim3(grid), dim3(FP_THREADS), shm, s,
+    hipLaunchKernelGGL((k_for_proj_diff_batch<S>), dim3(grid), dim3(FP_THREADS), shm, s,
                        items, n_items, total, counters);
-    hipLaunchKernelGGL((k_for_proj_diff_batch<S, true>), dim3(grid), dim3(FP_THREADS), shm, s,
-                       items, n_items, total, counters + 1);
 }
 
 void launch_fp_batch(int storage, const FpItem* items, int n_items, int total, int* counters,
@@ -1080,7 +1125,7 @@ static int batch_blocks_t(size_t fp_shm, bool fp)
     fp_batch_attrs<S>();
     int nb = 0;
     hipError_t e = fp ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                            &nb, k_for_proj_diff_batch<S, false>, FP_THREADS, fp_shm)
+                            &nb, k_for_proj_diff_batch<S>, FP_THREADS, fp_shm)
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_back_proj_batch<S>,
                                                                      BP_TX, 0);
     return (e == hipSuccess && nb > 0) ? nb : 1;

@@ -183,6 +183,7 @@ static atr_status run_round(atr_server* s)
             e.a = bp_args(p, p.d_sino, p.n, p.d_lists, n_all, 1.0f, true);
             e.item0 = tot_bp;
             e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
+            e.shape = bp_shape(p.bp_tz, p.bp_chunk);
             tot_bp += p.vol_y * e.n_items_y;
             s->h_bp[k][n_bp++] = e;
         } else if (r.step <= r.last) {
@@ -202,6 +203,7 @@ static atr_status run_round(atr_server* s)
                               false);
                 e.item0 = tot_bp;
                 e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
+                e.shape = bp_shape(p.bp_tz, p.bp_chunk);
                 tot_bp += p.vol_y * e.n_items_y;
                 s->h_bp[k][n_bp++] = e;
             }

@@ -26,12 +26,19 @@ template <typename S>
 __device__ __forceinline__ void st(S* p, float v);
 template <>
 __device__ __forceinline__ void st<float>(float* p, float v) { *p = v; }
+// FP16 stores saturate at the largest finite half on both sides.  Sentinels
+// (-1e30) then become -65504, which the kernels still classify as "empty"
+// through the storage-aware threshold; a value above 65504 stays finite
+// (65504) instead of becoming +inf, which would turn every difference and
+// interpolation pair that reads it into inf or NaN.
+__device__ __forceinline__ float half_saturate(float v)
+{
+    return fminf(fmaxf(v, -65504.0f), 65504.0f);
+}
 template <>
 __device__ __forceinline__ void st<_Float16>(_Float16* p, float v)
 {
-    // Sentinels (-1e30) do not fit FP16; they saturate to -65504, which the
-    // kernels still classify as "empty" through the storage-aware threshold.
-    *p = (_Float16)fmaxf(v, -65504.0f);
+    *p = (_Float16)half_saturate(v);
 }
 template <>
 __device__ __forceinline__ void st<bf16_t>(bf16_t* p, float v)
@@ -71,7 +78,7 @@ __device__ __forceinline__ float ldg(const ATR_GLOBAL_AS bf16_t* p)
 __device__ __forceinline__ void stg(ATR_GLOBAL_AS float* p, float v) { *p = v; }
 __device__ __forceinline__ void stg(ATR_GLOBAL_AS _Float16* p, float v)
 {
-    *p = (_Float16)fmaxf(v, -65504.0f);
+    *p = (_Float16)half_saturate(v);
 }
 __device__ __forceinline__ void stg(ATR_GLOBAL_AS bf16_t* p, float v)
 {

@@ -164,13 +164,19 @@ atr_status atr_execute_host(atr_plan* plan, const float* proj_host, float* vol_h
  * order whatever the shape; the FP32 rounding of the back-projection
  * coordinate (taken relative to the tile's window) and the order in which a
  * forward ray adds up its z-bands depend on the shape, at the 1e-7 level.
- * Outputs may be NULL.  A tuned plan with a non-default back-projection
- * shape cannot join a batch or be served. */
+ * Outputs may be NULL.  A candidate whose launch fails is dropped, never
+ * cached.  A plan that belongs to a batch (atr_batch_create until
+ * atr_batch_destroy) cannot be retuned: ATR_ERR_INVALID_ARG. */
 atr_status atr_plan_autotune(atr_plan* plan, void* stream, int* bp_tz, int* bp_chunk,
                              int* fp_band_rows);
 /* Sets the tile shapes by hand: bp_tz, bp_chunk in {8, 16}; fp_band_rows in
- * [2, the largest band that fits in LDS]. */
+ * [1, the largest band that fits in LDS].  Refused for a plan that belongs
+ * to a batch, as atr_plan_autotune. */
 atr_status atr_plan_set_tile_shape(atr_plan* plan, int bp_tz, int bp_chunk, int fp_band_rows);
+/* The forward-projection band heights (cell rows) the autotuner times for a
+ * slice of vol_x x vol_z, without a GPU: every one fits the LDS, none
+ * appears twice.  *n receives their number (at most 3); rows may be NULL. */
+atr_status atr_fp_band_candidates(int vol_x, int vol_z, int* rows, int cap, int* n);
 atr_status atr_plan_set_profiling(atr_plan* plan, int enable);
 atr_status atr_plan_kernel_stats(const atr_plan* plan, int kernel, int* launches,
                                  double* total_ms);

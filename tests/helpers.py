@@ -19,3 +19,37 @@ def small_series(seed=7, px=64, py=12, tilts=None, vol_z=32, n_ell=30, noise=0.0
                                        axis_min=amin, axis_max=amax, tilt_axis=axis,
                                        max_shift=shift, noise_frac=noise)
     return np.asarray(tilts, np.float32), proj, shifts
+
+
+def download_frames(dev, frame_shape, frames):
+    """Frames `frames` of a device array of [k][*frame_shape] float32 frames,
+    without copying the rest to the host."""
+    import ctypes
+
+    from aretomo3_amd import _lib as L
+    per = int(np.prod(frame_shape))
+    out = np.empty((len(frames),) + tuple(frame_shape), np.float32)
+    for i, f in enumerate(frames):
+        L.check(L.lib().atr_memcpy_d2h(out[i].ctypes.data, dev.ptr + 4 * per * int(f), 4 * per))
+    return out
+
+
+def fourier_ring_correlation(a, b, n_rings=32):
+    """Fourier ring correlation of two stacks of XZ slices [k][z][x], summed
+    over the slices, per ring of normalised radius (ring n_rings - 1 ends at
+    Nyquist)."""
+    fa = np.fft.rfft2(np.asarray(a, np.float64))
+    fb = np.fft.rfft2(np.asarray(b, np.float64))
+    nz, nx = fa.shape[1], a.shape[2]
+    kz = np.fft.fftfreq(nz)[:, None]
+    kx = np.fft.rfftfreq(nx)[None, :]
+    r = np.sqrt(kz ** 2 + kx ** 2) / 0.5
+    ring = np.minimum((r * n_rings).astype(int), n_rings).ravel()
+    num = np.zeros(n_rings + 1)
+    da = np.zeros(n_rings + 1)
+    db = np.zeros(n_rings + 1)
+    for k in range(fa.shape[0]):
+        num += np.bincount(ring, (fa[k] * np.conj(fb[k])).real.ravel(), n_rings + 1)
+        da += np.bincount(ring, (np.abs(fa[k]) ** 2).ravel(), n_rings + 1)
+        db += np.bincount(ring, (np.abs(fb[k]) ** 2).ravel(), n_rings + 1)
+    return (num / np.sqrt(np.maximum(da * db, 1e-300)))[:n_rings]

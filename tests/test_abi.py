@@ -203,3 +203,37 @@ def test_phantom_line_integral_of_one_ball():
     assert p.max() <= 2 * 5 * 1.0 + 1e-3
     assert p.max() >= 2 * 5 * 0.5 - 1e-3
     assert p.min() >= 0.0
+
+
+def band_candidates(vol_x, vol_z):
+    rows = (ctypes.c_int * 8)()
+    n = ctypes.c_int()
+    L.check(L.lib().atr_fp_band_candidates(vol_x, vol_z, rows, 8, ctypes.byref(n)))
+    return list(rows[:n.value])
+
+
+# largest band height whose bz + 1 rows of VolX + 1 float2 pairs fit 150 KiB,
+# capped at 17 (the forward projector's LDS budget), and at least 1
+def lds_limit(vol_x):
+    return min(17, max(1, (150 * 1024) // ((vol_x + 1) * 8) - 1))
+
+
+@pytest.mark.parametrize("vol_x,vol_z", [(512, 128), (1024, 256), (2048, 512), (4096, 40),
+                                         (6399, 64), (6400, 64), (8192, 256), (9599, 100)])
+def test_autotune_band_candidates_fit_the_lds(vol_x, vol_z):
+    c = band_candidates(vol_x, vol_z)
+    assert 1 <= len(c) <= 3 and len(set(c)) == len(c)
+    lim = lds_limit(vol_x)
+    assert all(1 <= b <= lim for b in c), (c, lim)
+    # every candidate's LDS fits the hardware's 160 KiB as well
+    assert all((b + 1) * (vol_x + 1) * 8 <= 160 * 1024 for b in c)
+    if vol_x >= 6400:
+        assert c == [1]
+
+
+def test_autotune_band_candidates_at_the_headline_size():
+    # VolZ 256 has 255 cell rows: 17 rows split them into 15 equal bands;
+    # 2/3 and 1/2 of 17 (11 and 8) need 24 and 32 bands, of at most 11 and 8
+    assert band_candidates(1024, 256) == [17, 11, 8]
+    with pytest.raises(L.AtrError):
+        L.check(L.lib().atr_fp_band_candidates(1024, 1, None, 0, ctypes.byref(ctypes.c_int())))
