@@ -87,7 +87,8 @@ class RequestInfo(ctypes.Structure):
     """Mirror of atr_request_info."""
     _fields_ = [("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
                 ("done_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
-                ("status", ctypes.c_int)]
+                ("status", ctypes.c_int), ("prepared_ms", ctypes.c_double),
+                ("compute_done_ms", ctypes.c_double)]
 
 
 class MrcInfo(ctypes.Structure):
@@ -180,8 +181,8 @@ def lib():
         "atr_server_destroy": (None, [vp]),
         "atr_host_alloc": (i, [ctypes.POINTER(vp), sz]),
         "atr_host_free": (i, [vp]),
-        "atr_plan_autotune": (i, [vp, vp, pi, pi, pi]),
-        "atr_plan_set_tile_shape": (i, [vp, i, i, i]),
+        "atr_plan_autotune": (i, [vp, vp, pi, pi, pi, pi]),
+        "atr_plan_set_tile_shape": (i, [vp, i, i, i, i]),
         "atr_fp_band_candidates": (i, [i, i, pi, i, pi]),
         "atr_aligned_size": (i, [i, i, f, pi, pi]),
         "atr_mrc_read_info": (i, [ctypes.c_char_p, ctypes.POINTER(MrcInfo)]),

@@ -36,23 +36,40 @@ atr_status hip_fail(hipError_t e, const char* what)
     return e == hipErrorOutOfMemory ? ATR_ERR_OUT_OF_MEMORY : ATR_ERR_HIP;
 }
 
+// Device memory of a plan: its allocator when it has one (the server's
+// arena), else hipMalloc / hipFree.
+static hipError_t plan_malloc(const Plan& p, void** dst, size_t b)
+{
+    return p.alloc ? p.alloc->alloc(dst, b) : hipMalloc(dst, b);
+}
+
+static void plan_mfree(const Plan& p, void* q)
+{
+    if (q == nullptr) return;
+    if (p.alloc) p.alloc->release(q);
+    else (void)hipFree(q);
+}
+
 template <class T>
-static atr_status upload(T** dst, const std::vector<T>& src, size_t& bytes)
+static atr_status upload(const Plan& p, T** dst, const std::vector<T>& src, size_t& bytes)
 {
     *dst = nullptr;
     if (src.empty()) return ATR_OK;
     size_t b = sizeof(T) * src.size();
-    ATR_HIP(hipMalloc((void**)dst, b));
-    ATR_HIP(hipMemcpy(*dst, src.data(), b, hipMemcpyHostToDevice));
+    ATR_HIP(plan_malloc(p, (void**)dst, b));
+    if (p.upload_stream_set)
+        ATR_HIP(hipMemcpyAsync(*dst, src.data(), b, hipMemcpyHostToDevice, p.upload_stream));
+    else
+        ATR_HIP(hipMemcpy(*dst, src.data(), b, hipMemcpyHostToDevice));
     bytes += b;
     return ATR_OK;
 }
 
-static atr_status dalloc(void** dst, size_t b, size_t& bytes)
+static atr_status dalloc(const Plan& p, void** dst, size_t b, size_t& bytes)
 {
     *dst = nullptr;
     if (b == 0) return ATR_OK;
-    ATR_HIP(hipMalloc(dst, b));
+    ATR_HIP(plan_malloc(p, dst, b));
     bytes += b;
     return ATR_OK;
 }
@@ -234,51 +251,67 @@ static size_t plan_bytes_estimate(const Plan& p)
 }
 
 template <class T>
-static atr_status reupload(T** dst, const std::vector<T>& src, size_t& bytes, size_t& old_bytes)
+static atr_status reupload(const Plan& p, T** dst, const std::vector<T>& src, size_t& bytes,
+                           size_t& old_bytes)
 {
     if (*dst) {
-        ATR_HIP(hipFree(*dst));
+        plan_mfree(p, *dst);
+        *dst = nullptr;
         bytes -= old_bytes;
     }
     old_bytes = sizeof(T) * src.size();
-    return upload(dst, src, bytes);
+    return upload(p, dst, src, bytes);
 }
 
-// The SART tables that depend on the tile shapes: back-projection tile
-// windows/classes for z-tiles of tz rows, forward-projection band tables and
-// wave schedules for bands of bz cell rows, and the per-step kernel
-// arguments.  Rebuilt by the autotuner for each candidate.
-static atr_status sart_tables(Plan& p, int tz, int chunk, int bz)
+// Host half of the SART tables that depend on the tile shapes: back-
+// projection tile windows/classes for z-tiles of tz rows, forward-projection
+// band tables and wave schedules for bands of bz cell rows, over all waves
+// or, with the band prefetch (pf), all but the loading wave.  No device work.
+static atr_status sart_tables_host(Plan& p, int tz, int chunk, int bz, int pf)
+{
+    Geometry& g = p.geo;
+    p.bp_tz = tz;
+    p.bp_chunk = chunk;
+    p.fp_pf = pf && fp_pf_supported(p.vol_x) ? 1 : 0;
+    build_bp_tiles(g.cos_sin, g.bp_lohi, p.n, p.vol_x, p.vol_z, 1.0, p.proj / 2.0, tz, g.bp_tiles);
+    if (!build_fp_tables(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, bz, g))
+        return fail(ATR_ERR_INVALID_ARG, "SART needs every tilt in (-90, 90) degrees");
+    std::vector<int>& sched = p.host.sched;
+    sched.clear();
+    p.subset_pair_off.clear();
+    const int waves = fp_waves(p.fp_pf);
+    for (int sb = 0; sb < p.n_subsets; sb++) {
+        p.subset_pair_off.push_back((int)sched.size());
+        const int f0 = p.subset_fp_off[sb], f1 = p.subset_fp_off[sb + 1];
+        fp_pair_schedule(g, p.subset_fp.data() + f0, f1 - f0, p.proj,
+                         fp_group(p.proj, f1 - f0, waves), sched, waves);
+    }
+    return ATR_OK;
+}
+
+// Device half: uploads the shape-dependent tables and the per-step kernel
+// arguments (which hold device pointers).
+static atr_status sart_tables_device(Plan& p)
 {
     Geometry& g = p.geo;
     size_t& B = p.device_bytes;
     atr_status st;
-    p.bp_tz = tz;
-    p.bp_chunk = chunk;
-    build_bp_tiles(g.cos_sin, g.bp_lohi, p.n, p.vol_x, p.vol_z, 1.0, p.proj / 2.0, tz, g.bp_tiles);
-    if (!build_fp_tables(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, bz, g))
-        return fail(ATR_ERR_INVALID_ARG, "SART needs every tilt in (-90, 90) degrees");
-    if ((st = reupload(&p.d_bp_tiles, g.bp_tiles, B, p.tab_bytes[0])) != ATR_OK) return st;
-    if ((st = reupload(&p.d_fp_range, g.fp_range, B, p.tab_bytes[1])) != ATR_OK) return st;
-    if ((st = reupload(&p.d_fp_geo_off, g.fp_geo_off, B, p.tab_bytes[2])) != ATR_OK) return st;
-    if ((st = reupload(&p.d_fp_len, g.fp_len, B, p.tab_bytes[3])) != ATR_OK) return st;
-    if ((st = reupload(&p.d_fp_geo, g.fp_geo, B, p.tab_bytes[4])) != ATR_OK) return st;
-    if ((st = reupload(&p.d_fp_desc, g.fp_desc, B, p.tab_bytes[5])) != ATR_OK) return st;
-    std::vector<int> sched;
-    p.subset_pair_off.clear();
-    for (int sb = 0; sb < p.n_subsets; sb++) {
-        p.subset_pair_off.push_back((int)sched.size());
-        const int f0 = p.subset_fp_off[sb], f1 = p.subset_fp_off[sb + 1];
-        fp_pair_schedule(g, p.subset_fp.data() + f0, f1 - f0, p.proj, fp_group(p.proj, f1 - f0),
-                         sched);
-    }
-    if ((st = reupload(&p.d_fp_pairs, sched, B, p.tab_bytes[6])) != ATR_OK) return st;
+    if ((st = reupload(p, &p.d_bp_tiles, g.bp_tiles, B, p.tab_bytes[0])) != ATR_OK) return st;
+    if ((st = reupload(p, &p.d_fp_range, g.fp_range, B, p.tab_bytes[1])) != ATR_OK) return st;
+    if ((st = reupload(p, &p.d_fp_geo_off, g.fp_geo_off, B, p.tab_bytes[2])) != ATR_OK) return st;
+    if ((st = reupload(p, &p.d_fp_len, g.fp_len, B, p.tab_bytes[3])) != ATR_OK) return st;
+    if ((st = reupload(p, &p.d_fp_geo, g.fp_geo, B, p.tab_bytes[4])) != ATR_OK) return st;
+    if ((st = reupload(p, &p.d_fp_desc, g.fp_desc, B, p.tab_bytes[5])) != ATR_OK) return st;
+    if ((st = reupload(p, &p.d_fp_pairs, p.host.sched, B, p.tab_bytes[6])) != ATR_OK) return st;
     // kernel arguments of every SART step, read by the kernels from device
     // memory: [0] initial back-projection, [1 + s] subset s
     const int n_all = (int)p.all_tilts.size();
-    std::vector<BpArgs> bpa;
+    // kept in the plan's host tables: a stream-ordered upload reads them later
+    std::vector<BpArgs>& bpa = p.host.bp_args;
+    std::vector<FpArgs>& fpa = p.host.fp_args;
+    bpa.clear();
+    fpa.clear();
     bpa.push_back(bp_args(p, p.d_sino, p.n, p.d_lists, n_all, 1.0f, true));
-    std::vector<FpArgs> fpa;
     for (int s = 0; s < p.n_subsets; s++) {
         const int b0 = p.subset_off[s], b1 = p.subset_off[s + 1];
         const int f0 = p.subset_fp_off[s], f1 = p.subset_fp_off[s + 1];
@@ -286,170 +319,214 @@ static atr_status sart_tables(Plan& p, int tz, int chunk, int bz)
                               false));
         fpa.push_back(fp_args(p, p.d_fp_lists + f0, f1 - f0, p.d_fp_pairs + p.subset_pair_off[s]));
     }
-    if ((st = reupload(&p.d_bp_args, bpa, B, p.tab_bytes[7])) != ATR_OK) return st;
-    if ((st = reupload(&p.d_fp_args, fpa, B, p.tab_bytes[8])) != ATR_OK) return st;
+    if ((st = reupload(p, &p.d_bp_args, bpa, B, p.tab_bytes[7])) != ATR_OK) return st;
+    if ((st = reupload(p, &p.d_fp_args, fpa, B, p.tab_bytes[8])) != ATR_OK) return st;
     return ATR_OK;
 }
 
-static atr_status plan_build(Plan& p)
+// Rebuilds the shape-dependent tables (autotuner, atr_plan_set_tile_shape).
+static atr_status sart_tables(Plan& p, int tz, int chunk, int bz, int pf)
+{
+    atr_status st = sart_tables_host(p, tz, chunk, bz, pf);
+    if (st != ATR_OK) return st;
+    return sart_tables_device(p);
+}
+
+// Host half of a plan: every table the kernels read, computed on the host in
+// FP64 from the descriptor (and the arrays it points to).  No device work, so
+// it can run on any thread (the server builds plans while earlier series
+// are on the GPU).
+static atr_status plan_host(Plan& p)
 {
     const atr_series_desc& d = p.desc;
     Geometry& g = p.geo;
+    HostTables& h = p.host;
     cos_sin_table(d.tilt_deg, p.n, g.cos_sin);
-    size_t& B = p.device_bytes;
-    B = 0;
-    std::vector<float2> cs(p.n);
-    for (int t = 0; t < p.n; t++) cs[t] = make_float2(g.cos_sin[2 * t], g.cos_sin[2 * t + 1]);
-    atr_status st;
-    if ((st = upload(&p.d_cs, cs, B)) != ATR_OK) return st;
-    std::vector<unsigned char> excl(p.n, 0);
+    h.cs.resize(p.n);
+    for (int t = 0; t < p.n; t++) h.cs[t] = make_float2(g.cos_sin[2 * t], g.cos_sin[2 * t + 1]);
+    h.excl.assign(p.n, 0);
     if (d.exclude)
-        for (int t = 0; t < p.n; t++) excl[t] = d.exclude[t] ? 1 : 0;
-    if ((st = upload(&p.d_excl, excl, B)) != ATR_OK) return st;
-
+        for (int t = 0; t < p.n; t++) h.excl[t] = d.exclude[t] ? 1 : 0;
     if (p.prestage) {
         // Correct/GCorrPatchShift.cu:104,142: angle = axis * (3.141592654f/180)
         // in float, cos/sin in double.
         const float d2r = 3.141592654f / 180.0f;
-        std::vector<double> acs(2 * (size_t)p.n);
-        std::vector<float> sh(2 * (size_t)p.n, 0.0f);
+        h.axis_cs.assign(2 * (size_t)p.n, 0.0);
+        h.shifts.assign(2 * (size_t)p.n, 0.0f);
         for (int t = 0; t < p.n; t++) {
             float ang = d.tilt_axis_deg[t] * d2r;
-            acs[2 * t] = std::cos((double)ang);
-            acs[2 * t + 1] = std::sin((double)ang);
+            h.axis_cs[2 * t] = std::cos((double)ang);
+            h.axis_cs[2 * t + 1] = std::sin((double)ang);
             if (d.shift_xy) {
-                sh[2 * t] = d.shift_xy[2 * t];
-                sh[2 * t + 1] = d.shift_xy[2 * t + 1];
+                h.shifts[2 * t] = d.shift_xy[2 * t];
+                h.shifts[2 * t + 1] = d.shift_xy[2 * t + 1];
             }
         }
-        if ((st = upload(&p.d_axis_cs, acs, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_shifts, sh, B)) != ATR_OK) return st;
-        if ((st = dalloc((void**)&p.d_aligned, sizeof(float) * p.n * (size_t)p.aln_x * p.aln_y, B)) != ATR_OK) return st;
-        if ((st = dalloc((void**)&p.d_row_stats, sizeof(double) * 2 * p.n * (size_t)p.aln_y, B)) != ATR_OK) return st;
-        if ((st = dalloc((void**)&p.d_tilt_stats, sizeof(double) * 2 * p.n, B)) != ATR_OK) return st;
-        if (p.fill) {
-            std::vector<int> tbl;
-            fill_table(d.fill_seed, p.aln_x * p.aln_y, tbl);
-            if ((st = upload(&p.d_fill_aln, tbl, B)) != ATR_OK) return st;
-        }
-        if (p.n_patches > 0) {
-            std::vector<float> lp(d.local_params, d.local_params + (size_t)p.n * 5 * p.n_patches);
-            if ((st = upload(&p.d_local, lp, B)) != ATR_OK) return st;
-        }
+        if (p.fill) fill_table(d.fill_seed, p.aln_x * p.aln_y, h.fill_aln);
+        if (p.n_patches > 0)
+            h.local.assign(d.local_params, d.local_params + (size_t)p.n * 5 * p.n_patches);
     }
     if (p.binned) {
-        const int kx = p.bin_x / 2 + 1;
-        if ((st = dalloc((void**)&p.d_binned, sizeof(float) * p.n * (size_t)p.bin_x * p.bin_y, B)) != ATR_OK) return st;
-        if ((st = dalloc((void**)&p.d_bin_t, sizeof(float2) * p.n * (size_t)kx * p.aln_y, B)) != ATR_OK) return st;
-        if ((st = dalloc((void**)&p.d_bin_u, sizeof(float2) * p.n * (size_t)kx * p.bin_y, B)) != ATR_OK) return st;
         FftDesc* fds[4] = {&p.fft_bin_ax, &p.fft_bin_ay, &p.fft_bin_by, &p.fft_bin_bx};
         const int sizes[4] = {p.aln_x, p.aln_y, p.bin_y, p.bin_x};
         for (int i = 0; i < 4; i++) {
             make_fft(sizes[i], *fds[i]);
-            if ((st = upload(&p.d_tw_bin[i], twiddles(sizes[i]), B)) != ATR_OK) return st;
+            h.tw_bin[i] = twiddles(sizes[i]);
+        }
+    }
+    if (p.method == ATR_METHOD_SART) {
+        build_bp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.bp_lohi);
+        h.lists = p.all_tilts;
+        h.lists.insert(h.lists.end(), p.subset_tilts.begin(), p.subset_tilts.end());
+        fill_table(d.fill_seed ^ 0x5A5A5A5A5A5A5A5Aull, p.vol_x * p.vol_z, h.fill_vol);
+        return sart_tables_host(p, p.bp_tz, p.bp_chunk,
+                                fp_balanced_rows(fp_band_rows(p.vol_x), p.vol_z), 0);
+    }
+    build_wbp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.wbp_lohi);
+    // 2x-grid back-projection: tile windows/classes (excluded tilts are left
+    // out of the list, GBackProjWbp.cu:31)
+    build_bp_tiles(g.cos_sin, g.wbp_lohi, p.n, 2 * p.vol_x, 2 * p.vol_z, 0.5,
+                   (double)(p.proj / 2) + 0.5, BP_TZ, h.wbp_tiles);
+    h.filter = ramp_filter(p.proj);
+    make_fft(p.proj, p.fft_proj);
+    make_fft(2 * p.vol_x, p.fft_x2);
+    make_fft(2 * p.vol_z, p.fft_z2);
+    make_fft(p.vol_z, p.fft_z);
+    make_fft(p.vol_x, p.fft_x);
+    h.tw_proj = twiddles(p.proj);
+    h.tw_x2 = twiddles(2 * p.vol_x);
+    h.tw_z2 = twiddles(2 * p.vol_z);
+    h.tw_z = twiddles(p.vol_z);
+    h.tw_x = twiddles(p.vol_x);
+    return ATR_OK;
+}
+
+// Device half: allocations (through the plan's allocator) and uploads of the
+// host tables, then the kernel arguments that hold device pointers.  The
+// host copies of the uploaded tables are released afterwards.
+static atr_status plan_device(Plan& p)
+{
+    HostTables& h = p.host;
+    Geometry& g = p.geo;
+    size_t& B = p.device_bytes;
+    B = 0;
+    atr_status st;
+    if ((st = upload(p, &p.d_cs, h.cs, B)) != ATR_OK) return st;
+    if ((st = upload(p, &p.d_excl, h.excl, B)) != ATR_OK) return st;
+    if (p.prestage) {
+        if ((st = upload(p, &p.d_axis_cs, h.axis_cs, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_shifts, h.shifts, B)) != ATR_OK) return st;
+        if ((st = dalloc(p, (void**)&p.d_aligned, sizeof(float) * p.n * (size_t)p.aln_x * p.aln_y, B)) != ATR_OK) return st;
+        if ((st = dalloc(p, (void**)&p.d_row_stats, sizeof(double) * 2 * p.n * (size_t)p.aln_y, B)) != ATR_OK) return st;
+        if ((st = dalloc(p, (void**)&p.d_tilt_stats, sizeof(double) * 2 * p.n, B)) != ATR_OK) return st;
+        if (p.fill && (st = upload(p, &p.d_fill_aln, h.fill_aln, B)) != ATR_OK) return st;
+        if (p.n_patches > 0 && (st = upload(p, &p.d_local, h.local, B)) != ATR_OK) return st;
+    }
+    if (p.binned) {
+        const int kx = p.bin_x / 2 + 1;
+        if ((st = dalloc(p, (void**)&p.d_binned, sizeof(float) * p.n * (size_t)p.bin_x * p.bin_y, B)) != ATR_OK) return st;
+        if ((st = dalloc(p, (void**)&p.d_bin_t, sizeof(float2) * p.n * (size_t)kx * p.aln_y, B)) != ATR_OK) return st;
+        if ((st = dalloc(p, (void**)&p.d_bin_u, sizeof(float2) * p.n * (size_t)kx * p.bin_y, B)) != ATR_OK) return st;
+        FftDesc* fds[4] = {&p.fft_bin_ax, &p.fft_bin_ay, &p.fft_bin_by, &p.fft_bin_bx};
+        for (int i = 0; i < 4; i++) {
+            if ((st = upload(p, &p.d_tw_bin[i], h.tw_bin[i], B)) != ATR_OK) return st;
             fds[i]->tw = p.d_tw_bin[i];
         }
     }
     size_t sb = storage_bytes(p.storage);
-    if ((st = dalloc(&p.d_sino, (size_t)p.vol_y * p.n * p.sino_stride * sb, B)) != ATR_OK) return st;
-    if ((st = dalloc((void**)&p.d_sflag, sizeof(int) * p.vol_y, B)) != ATR_OK) return st;
+    if ((st = dalloc(p, &p.d_sino, (size_t)p.vol_y * p.n * p.sino_stride * sb, B)) != ATR_OK) return st;
+    if ((st = dalloc(p, (void**)&p.d_sflag, sizeof(int) * p.vol_y, B)) != ATR_OK) return st;
 
     if (p.method == ATR_METHOD_SART) {
-        build_bp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.bp_lohi);
-        if ((st = upload(&p.d_bp_lohi, g.bp_lohi, B)) != ATR_OK) return st;
-        std::vector<TiltRow> lists(p.all_tilts);
-        lists.insert(lists.end(), p.subset_tilts.begin(), p.subset_tilts.end());
-        if ((st = upload(&p.d_lists, lists, B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_fp_lists, p.subset_fp, B)) != ATR_OK) return st;
-        std::vector<int> tbl;
-        fill_table(d.fill_seed ^ 0x5A5A5A5A5A5A5A5Aull, p.vol_x * p.vol_z, tbl);
-        if ((st = upload(&p.d_fill_vol, tbl, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_bp_lohi, g.bp_lohi, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_lists, h.lists, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_fp_lists, p.subset_fp, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_fill_vol, h.fill_vol, B)) != ATR_OK) return st;
         size_t vox = (size_t)p.vol_x * p.vol_y * p.vol_z;
-        if ((st = dalloc((void**)&p.d_vol, sizeof(float) * vox, B)) != ATR_OK) return st;
-        if ((st = dalloc(&p.d_diff, (size_t)p.vol_y * p.max_subset * p.sino_stride * sb, B)) != ATR_OK) return st;
-        if ((st = dalloc((void**)&p.d_vflag, sizeof(int) * p.vol_y, B)) != ATR_OK) return st;
-        if ((st = dalloc((void**)&p.d_vstats, sizeof(double) * 2 * p.vol_y, B)) != ATR_OK) return st;
-        p.device_bytes = B;
-        if ((st = sart_tables(p, p.bp_tz, p.bp_chunk,
-                              fp_balanced_rows(fp_band_rows(p.vol_x), p.vol_z))) != ATR_OK)
-            return st;
-        B = p.device_bytes;
+        if ((st = dalloc(p, (void**)&p.d_vol, sizeof(float) * vox, B)) != ATR_OK) return st;
+        if ((st = dalloc(p, &p.d_diff, (size_t)p.vol_y * p.max_subset * p.sino_stride * sb, B)) != ATR_OK) return st;
+        if ((st = dalloc(p, (void**)&p.d_vflag, sizeof(int) * p.vol_y, B)) != ATR_OK) return st;
+        if ((st = dalloc(p, (void**)&p.d_vstats, sizeof(double) * 2 * p.vol_y, B)) != ATR_OK) return st;
+        if ((st = sart_tables_device(p)) != ATR_OK) return st;
     } else {
-        build_wbp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.wbp_lohi);
-        if ((st = upload(&p.d_wbp_lohi, g.wbp_lohi, B)) != ATR_OK) return st;
-        {
-            // 2x-grid back-projection: tile windows/classes, the tilt list
-            // (excluded tilts left out, GBackProjWbp.cu:31), per-slab arguments
-            std::vector<int4> wt;
-            build_bp_tiles(g.cos_sin, g.wbp_lohi, p.n, 2 * p.vol_x, 2 * p.vol_z, 0.5,
-                           (double)(p.proj / 2) + 0.5, BP_TZ, wt);
-            if ((st = upload(&p.d_wbp_tiles, wt, B)) != ATR_OK) return st;
-            if ((st = upload(&p.d_lists, p.all_tilts, B)) != ATR_OK) return st;
-            if ((st = dalloc((void**)&p.d_wbp_grid,
-                             (size_t)p.wbp_slab * 4 * p.vol_z * p.vol_x * sizeof(float), B)) != ATR_OK)
-                return st;
-            std::vector<BpArgs> args;
-            for (int y0 = 0; y0 < p.vol_y; y0 += p.wbp_slab) {
-                BpArgs a{};
-                a.sino = (const char*)p.d_sino + (size_t)y0 * p.n * p.sino_stride * storage_bytes(p.storage);
-                a.rows_per_slice = p.n;
-                a.stride = p.sino_stride;
-                a.proj = p.proj;
-                a.list = p.d_lists;
-                a.n_list = (int)p.all_tilts.size();
-                a.cs = p.d_cs;
-                a.lohi = p.d_wbp_lohi;
-                a.tiles = p.d_wbp_tiles;
-                a.n_ztiles = (2 * p.vol_z + BP_TZ - 1) / BP_TZ;
-                a.n_xtiles = (2 * p.vol_x + BP_TX - 1) / BP_TX;
-                a.relax = 1.0f;
-                a.positivity = 0;
-                a.scale = 0.5f;
-                a.offset = (float)(p.proj / 2) + 0.5f;
-                a.sflag = p.d_sflag + y0;
-                a.vol = p.d_wbp_grid;
-                a.vol_x = 2 * p.vol_x;
-                a.vol_z = 2 * p.vol_z;
-                a.vflag = nullptr;
-                args.push_back(a);
-            }
-            if ((st = upload(&p.d_bp_args, args, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_wbp_lohi, g.wbp_lohi, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_wbp_tiles, h.wbp_tiles, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_lists, p.all_tilts, B)) != ATR_OK) return st;
+        if ((st = dalloc(p, (void**)&p.d_wbp_grid,
+                         (size_t)p.wbp_slab * 4 * p.vol_z * p.vol_x * sizeof(float), B)) != ATR_OK)
+            return st;
+        std::vector<BpArgs>& args = p.host.bp_args;
+        args.clear();
+        for (int y0 = 0; y0 < p.vol_y; y0 += p.wbp_slab) {
+            BpArgs a{};
+            a.sino = (const char*)p.d_sino + (size_t)y0 * p.n * p.sino_stride * storage_bytes(p.storage);
+            a.rows_per_slice = p.n;
+            a.stride = p.sino_stride;
+            a.proj = p.proj;
+            a.list = p.d_lists;
+            a.n_list = (int)p.all_tilts.size();
+            a.cs = p.d_cs;
+            a.lohi = p.d_wbp_lohi;
+            a.tiles = p.d_wbp_tiles;
+            a.n_ztiles = (2 * p.vol_z + BP_TZ - 1) / BP_TZ;
+            a.n_xtiles = (2 * p.vol_x + BP_TX - 1) / BP_TX;
+            a.relax = 1.0f;
+            a.positivity = 0;
+            a.scale = 0.5f;
+            a.offset = (float)(p.proj / 2) + 0.5f;
+            a.sflag = p.d_sflag + y0;
+            a.vol = p.d_wbp_grid;
+            a.vol_x = 2 * p.vol_x;
+            a.vol_z = 2 * p.vol_z;
+            a.vflag = nullptr;
+            args.push_back(a);
         }
-        if ((st = upload(&p.d_filter, ramp_filter(p.proj), B)) != ATR_OK) return st;
-        make_fft(p.proj, p.fft_proj);
-        make_fft(2 * p.vol_x, p.fft_x2);
-        make_fft(2 * p.vol_z, p.fft_z2);
-        make_fft(p.vol_z, p.fft_z);
-        make_fft(p.vol_x, p.fft_x);
-        if ((st = upload(&p.d_tw_proj, twiddles(p.proj), B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_tw_x2, twiddles(2 * p.vol_x), B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_tw_z2, twiddles(2 * p.vol_z), B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_tw_z, twiddles(p.vol_z), B)) != ATR_OK) return st;
-        if ((st = upload(&p.d_tw_x, twiddles(p.vol_x), B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_bp_args, args, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_filter, h.filter, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_tw_proj, h.tw_proj, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_tw_x2, h.tw_x2, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_tw_z2, h.tw_z2, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_tw_z, h.tw_z, B)) != ATR_OK) return st;
+        if ((st = upload(p, &p.d_tw_x, h.tw_x, B)) != ATR_OK) return st;
         p.fft_proj.tw = p.d_tw_proj;
         p.fft_x2.tw = p.d_tw_x2;
         p.fft_z2.tw = p.d_tw_z2;
         p.fft_z.tw = p.d_tw_z;
         p.fft_x.tw = p.d_tw_x;
         size_t kxn = p.vol_x / 2 + 1;
-        if ((st = dalloc((void**)&p.d_wbp_t, (size_t)p.wbp_slab * 2 * p.vol_z * kxn * sizeof(float2), B)) != ATR_OK) return st;
-        if ((st = dalloc((void**)&p.d_wbp_u, (size_t)p.wbp_slab * p.vol_z * kxn * sizeof(float2), B)) != ATR_OK) return st;
+        if ((st = dalloc(p, (void**)&p.d_wbp_t, (size_t)p.wbp_slab * 2 * p.vol_z * kxn * sizeof(float2), B)) != ATR_OK) return st;
+        if ((st = dalloc(p, (void**)&p.d_wbp_u, (size_t)p.wbp_slab * p.vol_z * kxn * sizeof(float2), B)) != ATR_OK) return st;
     }
+    // a stream-ordered upload still reads the host tables: the caller drops
+    // them once the stream has passed (plan_drop_host_tables)
+    if (!p.upload_stream_set) p.host = HostTables();
     return ATR_OK;
 }
 
+// Frees the plan's device memory and events; no device synchronisation.
 static void plan_free(Plan& p)
 {
-    void* ptrs[] = {p.d_cs, p.d_bp_lohi, p.d_bp_tiles, p.d_wbp_lohi, p.d_fp_range,
-                    p.d_fp_geo_off, p.d_fp_len, p.d_fp_geo, p.d_fp_desc, p.d_fp_pairs, p.d_lists, p.d_fp_lists,
-                    p.d_fill_aln, p.d_fill_vol, p.d_filter, p.d_tw_proj, p.d_tw_x2, p.d_tw_z2,
-                    p.d_tw_z, p.d_tw_x, p.d_axis_cs, p.d_shifts, p.d_excl, p.d_aligned,
-                    p.d_row_stats, p.d_tilt_stats, p.d_sino, p.d_diff, p.d_vol, p.d_sflag,
-                    p.d_vflag, p.d_vstats, p.d_wbp_t, p.d_wbp_u, p.d_bp_args, p.d_fp_args,
-                    p.d_wbp_grid, p.d_wbp_tiles, p.d_local, p.d_binned, p.d_bin_t, p.d_bin_u,
-                    p.d_tw_bin[0], p.d_tw_bin[1], p.d_tw_bin[2], p.d_tw_bin[3]};
-    for (void* q : ptrs)
-        if (q) (void)hipFree(q);
+    void** ptrs[] = {(void**)&p.d_cs, (void**)&p.d_bp_lohi, (void**)&p.d_bp_tiles,
+                     (void**)&p.d_wbp_lohi, (void**)&p.d_fp_range, (void**)&p.d_fp_geo_off,
+                     (void**)&p.d_fp_len, (void**)&p.d_fp_geo, (void**)&p.d_fp_desc,
+                     (void**)&p.d_fp_pairs, (void**)&p.d_lists, (void**)&p.d_fp_lists,
+                     (void**)&p.d_fill_aln, (void**)&p.d_fill_vol, (void**)&p.d_filter,
+                     (void**)&p.d_tw_proj, (void**)&p.d_tw_x2, (void**)&p.d_tw_z2,
+                     (void**)&p.d_tw_z, (void**)&p.d_tw_x, (void**)&p.d_axis_cs,
+                     (void**)&p.d_shifts, (void**)&p.d_excl, (void**)&p.d_aligned,
+                     (void**)&p.d_row_stats, (void**)&p.d_tilt_stats, (void**)&p.d_sino,
+                     (void**)&p.d_diff, (void**)&p.d_vol, (void**)&p.d_sflag, (void**)&p.d_vflag,
+                     (void**)&p.d_vstats, (void**)&p.d_wbp_t, (void**)&p.d_wbp_u,
+                     (void**)&p.d_bp_args, (void**)&p.d_fp_args, (void**)&p.d_wbp_grid,
+                     (void**)&p.d_wbp_tiles, (void**)&p.d_local, (void**)&p.d_binned,
+                     (void**)&p.d_bin_t, (void**)&p.d_bin_u, (void**)&p.d_tw_bin[0],
+                     (void**)&p.d_tw_bin[1], (void**)&p.d_tw_bin[2], (void**)&p.d_tw_bin[3]};
+    for (void** q : ptrs) {
+        plan_mfree(p, *q);
+        *q = nullptr;
+    }
+    for (size_t& b : p.tab_bytes) b = 0;
+    p.device_bytes = 0;
     for (auto& e : p.ev_pool) (void)hipEventDestroy(e);
     for (auto& e : p.ev_pending) {
         (void)hipEventDestroy(e.second.first);
@@ -458,6 +535,24 @@ static void plan_free(Plan& p)
     p.ev_pool.clear();
     p.ev_pending.clear();
 }
+
+// Plan construction in two halves, for callers that build the host tables
+// on one thread and allocate on another (serve.cpp).
+atr_status plan_prepare(const atr_series_desc* d, Plan& p)
+{
+    atr_status st = plan_sizes(d, p);
+    if (st == ATR_OK) st = plan_host(p);
+    return st;
+}
+
+atr_status plan_upload(Plan& p)
+{
+    atr_status st = plan_device(p);
+    if (st != ATR_OK) plan_free(p);
+    return st;
+}
+
+void plan_release(Plan& p) { plan_free(p); }
 
 // ---- profiling -----------------------------------------------------------
 static hipEvent_t take_event(Plan& p)
@@ -544,7 +639,7 @@ void sart_finish(Plan& p, float* out, hipStream_t s)
     });
 }
 
-static atr_status plan_execute(Plan& p, const float* in, float* out, int accumulate, hipStream_t s)
+atr_status plan_execute(Plan& p, const float* in, float* out, int accumulate, hipStream_t s)
 {
     if (in == nullptr || out == nullptr) return fail(ATR_ERR_INVALID_ARG, "NULL buffer");
     (void)hipGetLastError();
@@ -593,7 +688,7 @@ struct TuneKey {
     }
 };
 struct TuneChoice {
-    int tz, chunk, bz;
+    int tz, chunk, bz, pf;
 };
 static std::mutex g_tune_mu;
 static std::map<TuneKey, TuneChoice> g_tune_cache;
@@ -616,19 +711,21 @@ static float time_ms(hipStream_t s, int reps, F&& fn)
     return ms / reps;
 }
 
-static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_out, int* bz_out)
+static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_out, int* bz_out,
+                                int* pf_out)
 {
     if (p.method != ATR_METHOD_SART) {
         if (tz_out) *tz_out = 0;
         if (chunk_out) *chunk_out = 0;
         if (bz_out) *bz_out = 0;
+        if (pf_out) *pf_out = 0;
         return ATR_OK;
     }
     if (p.batch_refs > 0)
         return fail(ATR_ERR_INVALID_ARG,
                     "autotune: the plan belongs to a batch (destroy the batch first)");
     const TuneKey key{p.vol_x, p.vol_z, p.proj, p.n, p.n_subsets, p.storage};
-    TuneChoice best{p.bp_tz, p.bp_chunk, p.geo.fp_bz};
+    TuneChoice best{p.bp_tz, p.bp_chunk, p.geo.fp_bz, p.fp_pf};
     bool cached = false;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -658,7 +755,7 @@ static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_
         bool bp_ok = false;
         const int shapes[4][2] = {{16, 16}, {16, 8}, {8, 16}, {8, 8}};
         for (auto& sh : shapes) {
-            if ((st = sart_tables(p, sh[0], sh[1], bzs[0])) != ATR_OK) return st;
+            if ((st = sart_tables(p, sh[0], sh[1], bzs[0], 0)) != ATR_OK) return st;
             const float ms = time_ms(s, 3, [&] { launch_bp(p, has_bp ? 1 : 0, s); });
             if (hipGetLastError() != hipSuccess) continue;
             if (ms < best_bp) {
@@ -668,30 +765,41 @@ static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_
                 bp_ok = true;
             }
         }
+        // forward projection: the band heights of the direct-staging variant,
+        // then those of the band-prefetching one where VolX allows it
+        std::vector<std::pair<int, int>> fps;
+        for (int bz : bzs) fps.push_back({bz, 0});
+        if (fp_pf_supported(p.vol_x)) {
+            std::vector<int> pfz;
+            fp_band_candidates(p.vol_x, p.vol_z, pfz, 1);
+            for (int bz : pfz) fps.push_back({bz, 1});
+        }
         float best_fp = 1e30f;
         bool fp_ok = false;
-        for (int bz : bzs) {
-            if ((st = sart_tables(p, best.tz, best.chunk, bz)) != ATR_OK) return st;
+        for (auto& c : fps) {
+            if ((st = sart_tables(p, best.tz, best.chunk, c.first, c.second)) != ATR_OK) return st;
             const float ms = time_ms(s, 3, [&] { launch_fp_diff(p, 0, s); });
             if (hipGetLastError() != hipSuccess) continue;
             if (ms < best_fp) {
                 best_fp = ms;
-                best.bz = bz;
+                best.bz = c.first;
+                best.pf = c.second;
                 fp_ok = true;
             }
         }
         if (!bp_ok || !fp_ok) {
             // nothing launched: restore the default shape and report it
-            (void)sart_tables(p, BP_TZ, BP_CHUNK, bzs[0]);
+            (void)sart_tables(p, BP_TZ, BP_CHUNK, bzs[0], 0);
             return fail(ATR_ERR_HIP, "autotune: no candidate tile shape launched");
         }
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = best;
     }
-    if ((st = sart_tables(p, best.tz, best.chunk, best.bz)) != ATR_OK) return st;
+    if ((st = sart_tables(p, best.tz, best.chunk, best.bz, best.pf)) != ATR_OK) return st;
     if (tz_out) *tz_out = best.tz;
     if (chunk_out) *chunk_out = best.chunk;
     if (bz_out) *bz_out = best.bz;
+    if (pf_out) *pf_out = p.fp_pf;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "autotune");
     return ATR_OK;
@@ -741,7 +849,8 @@ atr_status atr_plan_create(const atr_series_desc* desc, atr_plan** plan)
     atr_plan* h = new (std::nothrow) atr_plan();
     if (!h) return fail(ATR_ERR_OUT_OF_MEMORY, "host allocation failed");
     atr_status st = plan_sizes(desc, h->p);
-    if (st == ATR_OK) st = plan_build(h->p);
+    if (st == ATR_OK) st = plan_host(h->p);
+    if (st == ATR_OK) st = plan_device(h->p);
     if (st != ATR_OK) {
         plan_free(h->p);
         delete h;
@@ -814,21 +923,27 @@ atr_status atr_execute_host(atr_plan* plan, const float* proj_host, float* vol_h
     return st;
 }
 
-atr_status atr_plan_set_tile_shape(atr_plan* plan, int bp_tz, int bp_chunk, int fp_rows)
+atr_status atr_plan_set_tile_shape(atr_plan* plan, int bp_tz, int bp_chunk, int fp_rows,
+                                   int fp_prefetch)
 {
     if (!plan) return fail(ATR_ERR_INVALID_ARG, "plan is NULL");
     Plan& p = plan->p;
     if (p.method != ATR_METHOD_SART) return fail(ATR_ERR_INVALID_ARG, "tile shapes are SART settings");
     if ((bp_tz != 8 && bp_tz != 16) || (bp_chunk != 8 && bp_chunk != 16))
         return fail(ATR_ERR_INVALID_ARG, "bp_tz and bp_chunk must be 8 or 16");
-    if (fp_rows < 1 || fp_rows > fp_band_rows(p.vol_x))
+    const int pf = fp_prefetch ? 1 : 0;
+    if (pf && !fp_pf_supported(p.vol_x))
+        return fail(ATR_ERR_INVALID_ARG, "the band-prefetching forward projector needs VolX a "
+                                         "multiple of 256 (VolX " + std::to_string(p.vol_x) + ")");
+    const int lim = fp_band_rows(p.vol_x, pf);
+    if (fp_rows < 1 || fp_rows > lim)
         return fail(ATR_ERR_INVALID_ARG,
-                    "fp band rows outside [1, " + std::to_string(fp_band_rows(p.vol_x)) +
-                        "] (the LDS limit at VolX " + std::to_string(p.vol_x) + ")");
+                    "fp band rows outside [1, " + std::to_string(lim) + "] (the LDS limit at VolX " +
+                        std::to_string(p.vol_x) + ")");
     if (p.batch_refs > 0)
         return fail(ATR_ERR_INVALID_ARG,
                     "set_tile_shape: the plan belongs to a batch (destroy the batch first)");
-    return sart_tables(p, bp_tz, bp_chunk, fp_rows);
+    return sart_tables(p, bp_tz, bp_chunk, fp_rows, pf);
 }
 
 atr_status atr_fp_band_candidates(int vol_x, int vol_z, int* rows, int cap, int* n)
@@ -844,10 +959,11 @@ atr_status atr_fp_band_candidates(int vol_x, int vol_z, int* rows, int cap, int*
 }
 
 atr_status atr_plan_autotune(atr_plan* plan, void* stream, int* bp_tz, int* bp_chunk,
-                             int* fp_band_rows_out)
+                             int* fp_band_rows_out, int* fp_prefetch)
 {
     if (!plan) return fail(ATR_ERR_INVALID_ARG, "plan is NULL");
-    return plan_autotune(plan->p, (hipStream_t)stream, bp_tz, bp_chunk, fp_band_rows_out);
+    return plan_autotune(plan->p, (hipStream_t)stream, bp_tz, bp_chunk, fp_band_rows_out,
+                         fp_prefetch);
 }
 
 atr_status atr_plan_set_profiling(atr_plan* plan, int enable)

@@ -35,6 +35,13 @@ constexpr int FP_MAXP = 16;
 constexpr int FP_LDS_BUDGET = 150 * 1024;
 constexpr int FP_BZ_MAX = 17;                // cell rows per band at most
 constexpr int FP_STAGE_ROWS = 9;              // rows a thread loads before forming pairs
+// Band-prefetching variant: the last wave streams the next band's volume rows
+// into LDS with direct global->LDS loads while the other waves sample the
+// current band; the raw rows sit after the pair rows.  Needs VolX % 256 == 0
+// (a wave's load instruction covers 256 floats of one row).
+constexpr int FP_LDS_PF = 160 * 1024;
+inline int fp_waves(int pf) { return pf ? FP_WAVES - 1 : FP_WAVES; }
+inline bool fp_pf_supported(int vol_x) { return vol_x >= 256 && vol_x % 256 == 0; }
 
 enum Kernel {
     K_BP = 0,
@@ -75,7 +82,37 @@ struct Geometry {
 struct BpArgs;
 struct FpArgs;
 
+// Device memory source of a plan: hipMalloc / hipFree when a plan has none,
+// the server's arena (which never returns memory to HIP while it serves, so
+// releasing a plan synchronises nothing) when it has one.
+struct DevAlloc {
+    virtual ~DevAlloc() = default;
+    virtual hipError_t alloc(void** ptr, size_t bytes) = 0;
+    virtual void release(void* ptr) = 0;
+};
+
+// Host copies of the tables a plan uploads, built before any device work
+// (plan_prepare) and released once uploaded (plan_upload).
+struct HostTables {
+    std::vector<float2> cs;
+    std::vector<unsigned char> excl;
+    std::vector<double> axis_cs;
+    std::vector<float> shifts, local, filter;
+    std::vector<int> fill_aln, fill_vol, sched;
+    std::vector<float2> tw_bin[4], tw_proj, tw_x2, tw_z2, tw_z, tw_x;
+    std::vector<TiltRow> lists;
+    std::vector<int4> wbp_tiles;
+    std::vector<BpArgs> bp_args;   // kernel arguments of every step
+    std::vector<FpArgs> fp_args;
+};
+
 struct Plan {
+    DevAlloc* alloc = nullptr;   // not owned; NULL = hipMalloc / hipFree
+    // table uploads ordered on this stream (hipMemcpyAsync; the host tables
+    // must then live until the stream has passed them), else synchronous
+    bool upload_stream_set = false;
+    hipStream_t upload_stream = nullptr;
+    HostTables host;
     atr_series_desc desc{};
     // sizes
     int n = 0;            // tilts
@@ -144,6 +181,7 @@ struct Plan {
     int* d_vflag = nullptr;           // [vol_y] volume slice holds sentinels
     double* d_vstats = nullptr;       // [vol_y][2] fill statistics of volume slices
     int bp_tz = BP_TZ, bp_chunk = BP_CHUNK;   // back-projection tile shape (autotuned)
+    int fp_pf = 0;                    // forward projector: band-prefetching variant (autotuned)
     // batches holding this plan: their item arrays and captured graphs point
     // at its tables, so the tables must not be rebuilt while this is > 0
     int batch_refs = 0;
@@ -203,6 +241,7 @@ struct FpArgs {
     int n, stride;
     void* diff;
     int diff_rows;
+    int pf;                    // band-prefetching variant (fp_waves(pf) sampling waves)
 };
 // Entries of the batched (work-queue) launches; item0 = first queue index.
 // shape: the series' back-projection tile shape, bp_shape(bp_tz, bp_chunk).
@@ -247,6 +286,16 @@ inline StageSource stage_source(const Plan& p, const float* in)
     return {in, p.in_tilt_stride, p.in_row_stride, false};
 }
 
+// ---- plan construction in two halves (api.cpp) -----------------------------
+// plan_prepare: validation, sizes and every host table (FP64 geometry, fill
+// tables, schedules); no device work.  plan_upload: device allocations
+// through p.alloc and the uploads; on failure what was allocated is freed.
+// plan_release: frees the device memory, without synchronising.
+atr_status plan_prepare(const atr_series_desc* d, Plan& p);
+atr_status plan_upload(Plan& p);
+void plan_release(Plan& p);
+atr_status plan_execute(Plan& p, const float* in, float* out, int accumulate, hipStream_t s);
+
 // ---- SART stages shared by single-plan and batched execution (api.cpp) ----
 atr_status sart_prepare(Plan& p, const float* in, float* out, int accumulate, hipStream_t s);
 void sart_finish(Plan& p, float* out, hipStream_t s);
@@ -271,12 +320,12 @@ void build_wbp_intervals(const std::vector<float>& cs, int n, int vol_x, int vol
                          std::vector<int2>& lohi);
 bool build_fp_tables(const std::vector<float>& cs, int n, int vol_x, int vol_z, int proj,
                      int bz, Geometry& g);
-int fp_band_rows(int vol_x);
+int fp_band_rows(int vol_x, int pf = 0);
 int fp_balanced_rows(int bz_max, int vol_z);
-void fp_band_candidates(int vol_x, int vol_z, std::vector<int>& rows);
-int fp_group(int proj, int n_list);
+void fp_band_candidates(int vol_x, int vol_z, std::vector<int>& rows, int pf = 0);
+int fp_group(int proj, int n_list, int n_waves = FP_WAVES);
 void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, int group,
-                      std::vector<int>& tab);
+                      std::vector<int>& tab, int n_waves = FP_WAVES);
 void split_items(int n_items, int n_splits, std::vector<int>& starts, std::vector<int>& sizes);
 void fill_table(uint64_t seed, int n, std::vector<int>& table);
 void aligned_size(int raw_x, int raw_y, float axis_deg, int* ax, int* ay);
@@ -300,7 +349,7 @@ int wbp_col_block(int vol_z);
 BpArgs bp_args(const Plan& p, const void* sino, int rows, const TiltRow* list, int n_list,
                float relax, bool check_sent);
 FpArgs fp_args(const Plan& p, const int* fp_list, int n_list, const int* sched);
-size_t fp_lds_bytes(int bz, int vol_x);
+size_t fp_lds_bytes(int bz, int vol_x, int pf = 0);
 // step 0: initial back-projection, 1 + s: subset s (arguments in p.d_bp_args)
 void launch_bp(const Plan& p, int step, hipStream_t s);
 void launch_bp_batch(int storage, const BpItem* items, int n_items, int total, int* counter,

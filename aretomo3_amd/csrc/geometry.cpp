@@ -350,10 +350,16 @@ std::vector<float> ramp_filter(int n)
 namespace atr {
 // Cell rows per forward-projection band: bz + 1 rows of (VolX + 1)
 // interpolation pairs must fit the LDS budget of one workgroup.
-int fp_band_rows(int vol_x)
+int fp_band_rows(int vol_x, int pf)
 {
-    int rows = FP_LDS_BUDGET / ((vol_x + 1) * (int)sizeof(float2));
-    return std::min(FP_BZ_MAX, std::max(1, rows - 1));
+    if (!pf) {
+        int rows = FP_LDS_BUDGET / ((vol_x + 1) * (int)sizeof(float2));
+        return std::min(FP_BZ_MAX, std::max(1, rows - 1));
+    }
+    // pair rows and raw rows together (fp_lds_bytes) within FP_LDS_PF
+    int bz = FP_BZ_MAX;
+    while (bz > 1 && fp_lds_bytes(bz, vol_x, 1) > (size_t)FP_LDS_PF) bz--;
+    return bz;
 }
 
 // The VolZ - 1 cell rows split into equal bands of at most bz_max rows: a
@@ -370,9 +376,9 @@ int fp_balanced_rows(int bz_max, int vol_z)
 // about 2/3 and 1/2 of it, each balanced over the slice's cell rows, none
 // above the LDS limit, no height twice.  At VolX >= 6400 only one row fits
 // (rows of VolX + 1 pairs), and the list is {1}.
-void fp_band_candidates(int vol_x, int vol_z, std::vector<int>& rows)
+void fp_band_candidates(int vol_x, int vol_z, std::vector<int>& rows, int pf)
 {
-    const int bzmax = fp_band_rows(vol_x);
+    const int bzmax = fp_band_rows(vol_x, pf);
     rows.clear();
     for (int want : {bzmax, bzmax * 2 / 3, bzmax / 2}) {
         const int bz = std::min(bzmax, fp_balanced_rows(std::max(1, want), vol_z));
@@ -385,10 +391,10 @@ void fp_band_candidates(int vol_x, int vol_z, std::vector<int>& rows)
 // Tilts per forward-projection workgroup for a list of n_list tilts: at most
 // the FP_WAVES * FP_MAXP pair slots of 64-ray tiles, and the list split into
 // groups of equal size (13 tilts with room for 8 -> 7 + 6, not 8 + 5).
-int fp_group(int proj, int n_list)
+int fp_group(int proj, int n_list, int n_waves)
 {
     const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
-    const int cap = std::max(1, FP_WAVES * FP_MAXP / ntile);
+    const int cap = std::max(1, n_waves * FP_MAXP / ntile);
     const int groups = (std::max(n_list, 1) + cap - 1) / cap;
     return (std::max(n_list, 1) + groups - 1) / groups;
 }
@@ -401,9 +407,11 @@ int fp_group(int proj, int n_list)
 // Cost of a pair in a band = the samples its descriptor covers, masked
 // windows weighted 1.4, plus a fixed 8 per non-empty descriptor.
 // tab gets one block of FP_WAVES * FP_MAXP entries per group of `group`
-// tilts: entry [w * FP_MAXP + i] = (local tilt << 16) | tile, or -1.
+// tilts: entry [w * FP_MAXP + i] = (local tilt << 16) | tile, or -1.  Only
+// waves 0 .. n_waves - 1 get pairs (the band-prefetching projector keeps its
+// last wave for the loads).
 void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, int group,
-                      std::vector<int>& tab)
+                      std::vector<int>& tab, int n_waves)
 {
     const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
     const int nb = g.fp_nbands;
@@ -425,18 +433,18 @@ void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, 
         for (int i = 0; i < n_items; i++) order[i] = i;
         std::stable_sort(order.begin(), order.end(),
                          [&](int a, int b) { return total[a] > total[b]; });
-        std::vector<double> load((size_t)FP_WAVES * nb, 0.0);
-        std::vector<int> owner(n_items, -1), fill(FP_WAVES, 0);
+        std::vector<double> load((size_t)n_waves * nb, 0.0);
+        std::vector<int> owner(n_items, -1), fill(n_waves, 0);
         auto band_max = [&](int b) {
             double m = 0.0;
-            for (int w = 0; w < FP_WAVES; w++) m = std::max(m, load[(size_t)w * nb + b]);
+            for (int w = 0; w < n_waves; w++) m = std::max(m, load[(size_t)w * nb + b]);
             return m;
         };
         std::vector<double> bmax(nb, 0.0);
         for (int it : order) {
             int best = -1;
             double best_obj = 0.0, best_tot = 0.0;
-            for (int w = 0; w < FP_WAVES; w++) {
+            for (int w = 0; w < n_waves; w++) {
                 if (fill[w] >= FP_MAXP) continue;
                 double obj = 0.0, tot = 0.0;
                 for (int b = 0; b < nb; b++) {

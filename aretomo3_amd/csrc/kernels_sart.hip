@@ -615,7 +615,28 @@ __device__ __forceinline__ void fp_run(const char* __restrict__ band, int rowb_r
 // One (group of tilts, slice) work item.  Returns early, for the whole
 // block, when the slice's sentinel flag does not match SENT.  Ends after a
 // barrier, with the LDS free for the next item.
-template <typename S, bool SENT, int ROWB = 0>
+// PF: the band-prefetching variant.  Wave FP_WAVES-1 has no pairs; while the
+// other waves sample band b, it streams band b+1's volume rows into the raw
+// buffer after the pair rows with direct global->LDS loads (one wave
+// instruction = 256 floats of a row; no registers, and the sampling waves'
+// own loads never wait on them: the memory counter is per wave).  All waves
+// then form the pairs from the raw rows, LDS to LDS, with the same
+// arithmetic as the direct staging, so both variants give the same pairs.
+__device__ __forceinline__ void fp_raw_load(const ATR_GLOBAL_AS float* __restrict__ V, float* raw,
+                                            int vol_x, int zb, int rows, int lane)
+{
+    const int per_row = vol_x >> 8;
+    const int chunks = rows * per_row;
+    for (int c = 0; c < chunks; c++) {
+        const int r = c / per_row, k = c - r * per_row;
+        const ATR_GLOBAL_AS float* src = V + (size_t)(zb + r) * vol_x + k * 256 + lane * 4;
+        float* dst = raw + r * vol_x + k * 256;
+        __builtin_amdgcn_global_load_lds((const ATR_GLOBAL_AS void*)src,
+                                         (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+    }
+}
+
+template <typename S, bool SENT, int ROWB = 0, bool PF = false>
 __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ band, int gx, int y)
 {
     const gptr<const float> vol = as_gptr(A->vol);
@@ -660,18 +681,50 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
         }
     }
     const char* bandb = (const char*)band;
+    // PF: raw rows of the next band, from the first 256-byte boundary after
+    // the pair rows; loaded by the last wave
+    float* raw = (float*)(bandb + ((size_t)(bz + 1) * pitch * sizeof(float2) + 255) / 256 * 256);
+    const bool loader = PF && wave == FP_WAVES - 1;
     // Work items start at different bands, so that the band reloads of the
     // workgroups on the chip are spread out in time instead of all hitting
     // HBM at once.  (A ray's samples are summed band by band in this order.)
     const int rot = (int)(((unsigned)y * 5u + (unsigned)gx) % (unsigned)nb);
+    if (PF) {
+        if (loader) {
+            const int zb = rot * bz;
+            fp_raw_load(V, raw, vol_x, zb, min(bz, vol_z - 1 - zb) + 1, lane);
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        __syncthreads();
+    }
     for (int bi = 0; bi < nb; bi++) {
         const int b = bi + rot < nb ? bi + rot : bi + rot - nb;
         const int zb = b * bz;
         const int rows = min(bz, vol_z - 1 - zb) + 1;  // volume rows zb .. zb+rows-1
+        if (PF) {
+            // pairs from the raw rows (LDS to LDS), then the next band's loads
+            for (int col = tid; col < pitch; col += FP_THREADS) {
+                const int xi = col - 1;
+                const int xa = max(xi, 0), xb = min(xa + 1, vol_x - 1);
+                for (int r = 0; r < rows; r++) {
+                    const float va = raw[r * vol_x + xa], vb = raw[r * vol_x + xb];
+                    const float dx = vb - va;
+                    const float v0 = xi < 0 ? va - dx : va;
+                    band[r * pitch + col] = make_float2(SENT ? v0 : fmaf(-(float)col, dx, v0), dx);
+                }
+            }
+            __syncthreads();
+            if (loader && bi + 1 < nb) {
+                const int b1 = b + 1 < nb ? b + 1 : 0;
+                const int zb1 = b1 * bz;
+                fp_raw_load(V, raw, vol_x, zb1, min(bz, vol_z - 1 - zb1) + 1, lane);
+            }
+        }
         // Staging: a thread owns whole columns of interpolation pairs
         // (V[z][x], V[z][x+1]-V[z][x]); column 0 is the guard for x in [-1, 0).
         // Each half of the rows is loaded first (independent, coalesced
         // loads), then the pairs are formed from registers.
+        if (!PF)
         for (int col = tid; col < pitch; col += FP_THREADS) {
             const int xi = col - 1;                      // -1 .. VolX-1
             const int xa = max(xi, 0), xb = min(xa + 1, vol_x - 1);
@@ -698,7 +751,7 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
                 }
             }
         }
-        __syncthreads();
+        if (!PF) __syncthreads();
 #pragma unroll
         for (int i = 0; i < FP_MAXP; i++) {
             const int jt = pair_tab[i];
@@ -724,6 +777,8 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
                 }
             }
         }
+        // PF: the next band's rows are in LDS once the loader's loads are done
+        if (loader) __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
 #pragma unroll
@@ -747,11 +802,12 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
 
 // ROWB: the band row pitch in bytes, 8 (VolX + 1), when the launch fixes it
 // at compile time (VolX 512, 1024, 2048), else 0.
-template <typename S, bool SENT, int ROWB>
+// PF: the band-prefetching variant (VolX a multiple of 256).
+template <typename S, bool SENT, int ROWB, bool PF = false>
 __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(const FpArgs* __restrict__ a)
 {
     extern __shared__ float2 band[];
-    fp_block<S, SENT, ROWB>(as_cptr(a), band, blockIdx.x, blockIdx.y);
+    fp_block<S, SENT, ROWB, PF>(as_cptr(a), band, blockIdx.x, blockIdx.y);
 }
 
 // Forward projection + difference of many series in one persistent launch:
@@ -1023,7 +1079,8 @@ FpArgs fp_args(const Plan& p, const int* fp_list, int n_list, const int* sched)
     a.vflag = p.d_vflag;
     a.fp_list = fp_list;
     a.n_list = n_list;
-    a.group = fp_group(p.proj, n_list);
+    a.group = fp_group(p.proj, n_list, fp_waves(p.fp_pf));
+    a.pf = p.fp_pf;
     a.sched = sched;
     a.cs = p.d_cs;
     a.fp_range = p.d_fp_range;
@@ -1040,44 +1097,60 @@ FpArgs fp_args(const Plan& p, const int* fp_list, int n_list, const int* sched)
     return a;
 }
 
-size_t fp_lds_bytes(int bz, int vol_x)
+// Pair rows of a band; with the prefetch, then (from a 256-byte boundary) the
+// raw rows of the next band.
+static size_t fp_pair_bytes(int bz, int vol_x) { return (size_t)(bz + 1) * (vol_x + 1) * sizeof(float2); }
+
+size_t fp_lds_bytes(int bz, int vol_x, int pf)
 {
-    return (size_t)(bz + 1) * (vol_x + 1) * sizeof(float2);
+    const size_t pairs = fp_pair_bytes(bz, vol_x);
+    if (!pf) return pairs;
+    return (pairs + 255) / 256 * 256 + (size_t)(bz + 1) * vol_x * sizeof(float);
 }
 
 // Kernels with more than 64 KiB of dynamic LDS must say so once per device;
 // `done` holds one bit per device the attribute was set on (devices 64 and
 // up set it at every launch).
 template <typename K>
-static void allow_big_lds(K kernel, std::atomic<unsigned long long>& done)
+static void allow_big_lds(K kernel, std::atomic<unsigned long long>& done, int bytes = FP_LDS_BUDGET)
 {
     int dev = -1;
     const bool known = hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64;
     const unsigned long long bit = known ? 1ull << dev : 0ull;
     if (known && (done.load(std::memory_order_relaxed) & bit)) return;
     if (hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            FP_LDS_BUDGET) == hipSuccess && known)
+                            bytes) == hipSuccess && known)
         done.fetch_or(bit, std::memory_order_relaxed);
 }
 
-template <typename S, bool SENT, int ROWB>
+template <typename S, bool SENT, int ROWB, bool PF = false>
 static void launch_fp_k(dim3 grid, size_t shm, const FpArgs* a, hipStream_t s)
 {
     static std::atomic<unsigned long long> done{0};
-    allow_big_lds(k_for_proj_diff<S, SENT, ROWB>, done);
-    hipLaunchKernelGGL((k_for_proj_diff<S, SENT, ROWB>), grid, dim3(FP_THREADS), shm, s, a);
+    allow_big_lds(k_for_proj_diff<S, SENT, ROWB, PF>, done, PF ? FP_LDS_PF : FP_LDS_BUDGET);
+    hipLaunchKernelGGL((k_for_proj_diff<S, SENT, ROWB, PF>), grid, dim3(FP_THREADS), shm, s, a);
 }
 
 template <typename S>
 static void launch_fp_t(const Plan& p, int subset, hipStream_t s)
 {
     const int n_list = p.subset_fp_off[subset + 1] - p.subset_fp_off[subset];
-    const int group = fp_group(p.proj, n_list);
+    const int group = fp_group(p.proj, n_list, fp_waves(p.fp_pf));
     dim3 grid((n_list + group - 1) / group, p.vol_y);
-    const size_t shm = fp_lds_bytes(p.geo.fp_bz, p.vol_x);
+    const size_t shm = fp_lds_bytes(p.geo.fp_bz, p.vol_x, p.fp_pf);
     const FpArgs* a = p.d_fp_args + subset;
     // the sentinel-free slices (nearly all) take a variant with the band
     // pitch fixed at compile time for the common widths
+    if (p.fp_pf) {
+        switch (p.vol_x) {
+        case 512: launch_fp_k<S, false, 513 * 8, true>(grid, shm, a, s); break;
+        case 1024: launch_fp_k<S, false, 1025 * 8, true>(grid, shm, a, s); break;
+        case 2048: launch_fp_k<S, false, 2049 * 8, true>(grid, shm, a, s); break;
+        default: launch_fp_k<S, false, 0, true>(grid, shm, a, s); break;
+        }
+        launch_fp_k<S, true, 0, true>(grid, shm, a, s);
+        return;
+    }
     switch (p.vol_x) {
     case 512: launch_fp_k<S, false, 513 * 8>(grid, shm, a, s); break;
     case 1024: launch_fp_k<S, false, 1025 * 8>(grid, shm, a, s); break;

@@ -158,14 +158,15 @@ class Plan:
 
     def autotune(self, stream=None):
         """Times the candidate tile shapes on this plan and keeps the fastest;
-        returns (bp z rows, bp tilts per stage, fp band rows)."""
-        tz, ch, bz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        returns (bp z rows, bp tilts per stage, fp band rows, fp prefetch)."""
+        tz, ch, bz, pf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().atr_plan_autotune(self.handle, stream, ctypes.byref(tz), ctypes.byref(ch),
-                                      ctypes.byref(bz)))
-        return tz.value, ch.value, bz.value
+                                      ctypes.byref(bz), ctypes.byref(pf)))
+        return tz.value, ch.value, bz.value, pf.value
 
-    def set_tile_shape(self, bp_tz, bp_chunk, fp_band_rows):
-        check(lib().atr_plan_set_tile_shape(self.handle, bp_tz, bp_chunk, fp_band_rows))
+    def set_tile_shape(self, bp_tz, bp_chunk, fp_band_rows, fp_prefetch=0):
+        check(lib().atr_plan_set_tile_shape(self.handle, bp_tz, bp_chunk, fp_band_rows,
+                                            1 if fp_prefetch else 0))
 
     @property
     def volume_shape(self):
@@ -334,7 +335,12 @@ class Server:
         finally:
             proj, vol, wbp, keep = self._live.pop(rid)
         return vol, wbp, {"submit_ms": info.submit_ms, "admit_ms": info.admit_ms,
-                          "done_ms": info.done_ms, "latency_ms": info.latency_ms}
+                          "done_ms": info.done_ms, "latency_ms": info.latency_ms,
+                          "prepared_ms": info.prepared_ms,
+                          "compute_done_ms": info.compute_done_ms,
+                          "queue_ms": info.admit_ms - info.submit_ms,
+                          "gpu_ms": info.compute_done_ms - info.admit_ms,
+                          "copy_out_ms": info.done_ms - info.compute_done_ms}
 
     def stats(self):
         r, m = ctypes.c_uint64(), ctypes.c_uint64()

@@ -28,9 +28,14 @@ local shifts and binning, the re-projector, the thickness measurement and MRC
 I/O, each beside the FP64 oracle), small (a reduced SART case for quick checks;
 not a benchmark).
 
+The default run also carries a "serving" block: a short stream of the
+serving configuration's request mix (40 requests, WBP + SART 10 x 5 each,
+FP16 storage) at 4/s for p50/p99 latency and as a 20/s burst for throughput,
+beside the plain per-series FP32 path on the same requests.
+
 Prints one JSON line.  Options: --steps K --warmup W --gpus N (only 1),
 --config NAME, --no-cpu-baseline, --cpu-slices S, --copy-probe MIB,
---requests R and --rate L (serve).
+--requests R, --rate L and --max-active M (serve), --no-serve.
 """
 import argparse
 import json
@@ -99,6 +104,37 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
 
 
+def measured_copy_peak(mib=1024, reps=10):
+    """Stream-copy bandwidth of this GPU in the same run: the plain copy
+    kernel (16-byte loads and stores) moving `mib` MiB in and `mib` MiB out,
+    timed with HIP events over `reps` launches.  The roofline's peak stays
+    the published 8 TB/s; this is the attainable figure beside it."""
+    from aretomo3_amd import DeviceArray
+    from aretomo3_amd._lib import check, lib
+    import ctypes
+    nbytes = mib << 20
+    a, b = DeviceArray(nbytes), DeviceArray(nbytes)
+    check(lib().atr_memset(a.ptr, 0, nbytes))
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    check(lib().atr_event_create(ctypes.byref(ev0)))
+    check(lib().atr_event_create(ctypes.byref(ev1)))
+    check(lib().atr_copy_kernel(b.ptr, a.ptr, nbytes, None))   # warm-up
+    check(lib().atr_event_record(ev0, None))
+    for _ in range(reps):
+        check(lib().atr_copy_kernel(b.ptr, a.ptr, nbytes, None))
+    check(lib().atr_event_record(ev1, None))
+    ms = ctypes.c_float()
+    check(lib().atr_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)))
+    check(lib().atr_event_destroy(ev0))
+    check(lib().atr_event_destroy(ev1))
+    a.free()
+    b.free()
+    gbs = 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+    return {"kernel": "k_copy (16-byte loads/stores, grid-stride)",
+            "bytes_per_launch": 2 * nbytes, "gbs": round(gbs, 1),
+            "frac_of_8000": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def fourier_ring_correlation(a, b, n_rings=32):
     """Mean over slices of the Fourier ring correlation of two stacks of XZ
     slices [k][z][x], per ring of normalised radius; returns the curve."""
@@ -135,144 +171,263 @@ def describe(name, w, vol):
              alg, w["storage"].upper()))
 
 
-SERVE = dict(requests=24, rate=1.0, pool=8, seed=20261206, large_frac=0.1, iters=10, subsets=5)
+SERVE = dict(requests=500, rates="1,2,5,10,20", seed=20261206, large_frac=0.1, iters=10, subsets=5,
+             max_active=4)
+# the short serving stream of the default run: a C5-shaped stream sized to
+# keep the whole default command within a few minutes
+SERVE_DEFAULT = dict(requests=40, rate=4.0, burst_rate=20.0)
 
 
-def serve_bench(args):
-    """The serving stream: open-loop Poisson arrivals of SART(10 iterations) + WBP
-    requests of mixed sizes (tilts {31,41,51,61} over +-60 deg, projections
-    {512,768,1024}^2, thickness {128,256,384}, tilt axis uniform 80-95 deg, plus
-    ~10% large requests 2048^2 x 61 x 512), FP16 projection storage, admission by
-    the device-memory budget.  Reports tomograms/s, p50/p99 latency and
-    Gvoxel-tilt updates/s, beside the plain per-series FP32 path on the same
-    requests (series one after another, no batching)."""
-    from aretomo3_amd import DeviceArray, PinnedArray, Plan, Server, synchronize
-    from aretomo3_amd import _lib as L
-    from aretomo3_amd import phantom
-    c = dict(SERVE)
-    c["requests"] = args.requests or c["requests"]
-    c["rate"] = args.rate or c["rate"]
-    rng = np.random.default_rng(c["seed"])
-    # a pool of distinct series; requests draw from it (the GPU work of a
-    # request depends on its sizes, not on the phantom)
-    pool = []
-    for k in range(c["pool"]):
-        large = k == 0
-        n = 61 if large else int(rng.choice([31, 41, 51, 61]))
-        px = 2048 if large else int(rng.choice([512, 768, 1024]))
-        vz = 512 if large else int(rng.choice([128, 256, 384]))
+def serve_sizes(rng, count, large_frac):
+    """Per-request sizes of the serving configuration: tilts {31,41,51,61}
+    over +-60 deg, projections {512,768,1024}^2, thickness {128,256,384}, tilt
+    axis uniform 80-95 deg (as the batching configuration draws them), and
+    ~large_frac of the requests 2048^2 x 61 x 512."""
+    out = []
+    for _ in range(count):
+        if rng.random() < large_frac:
+            out.append((61, 2048, 512))
+        else:
+            out.append((int(rng.choice([31, 41, 51, 61])), int(rng.choice([512, 768, 1024])),
+                        int(rng.choice([128, 256, 384]))))
+    return out
+
+
+class ServePool:
+    """One seeded series per request size (the GPU work of a request depends
+    on its sizes, not on the phantom), held in pinned host memory, with the
+    plain per-series FP32 path timed on it: upload, WBP, SART, both volumes
+    copied back to pinned host memory, series after series."""
+
+    def __init__(self, seed, iters, subsets):
+        self.seed, self.iters, self.subsets = seed, iters, subsets
+        self.series, self.plain_s = {}, {}
+        self.out = None
+
+    def get(self, size):
+        from aretomo3_amd import PinnedArray, phantom
+        if size in self.series:
+            return self.series[size]
+        n, px, vz = size
+        k = len(self.series)
+        rng = np.random.default_rng(self.seed + 1000 + k)
         axis = float(rng.uniform(80.0, 95.0))
         tilts = np.linspace(-60.0, 60.0, n).astype(np.float32)
         t0 = time.time()
-        raw, shifts = phantom.tilt_series(c["seed"] + k, px, px, tilts, vz,
-                                          n_ellipsoids=int(rng.integers(200, 801)),
-                                          axis_min=4.0, axis_max=40.0, tilt_axis=axis, max_shift=3.0,
+        raw, shifts = phantom.tilt_series(self.seed + k, px, px, tilts, vz,
+                                          n_ellipsoids=int(rng.integers(200, 801)), axis_min=4.0,
+                                          axis_max=40.0, tilt_axis=axis, max_shift=3.0,
                                           noise_frac=0.10, n_threads=16)
         pin = PinnedArray(raw.shape)
         pin.array[...] = raw
-        kw = dict(tilt_axis=np.full(n, axis, np.float32), shifts=shifts, sart_iters=c["iters"],
-                  sart_subsets=c["subsets"], relax=0.3, subset_order=L.SUBSETS_INTERLEAVED,
-                  positivity=False, fill_seed=c["seed"] + k)
-        ax = px
-        vx = ax // 2 * 2
-        updates = float(vx) * px * vz * n * (1 + 2 * c["iters"]) + float(vx) * px * vz * n
-        pool.append(dict(pin=pin, tilts=tilts, vz=vz, kw=kw, large=large, updates=updates,
-                         shape=raw.shape))
-        print(f"[bench] serve pool {k}: {raw.shape} vz {vz} in {time.time() - t0:.1f}s",
-              file=sys.stderr, flush=True)
-    # request stream: ~large_frac of requests are the large series
-    picks = [0 if rng.random() < c["large_frac"] else int(rng.integers(1, c["pool"]))
-             for _ in range(c["requests"])]
-    gaps = rng.exponential(1.0 / c["rate"], c["requests"])
+        kw = dict(tilt_axis=np.full(n, axis, np.float32), shifts=shifts, sart_iters=self.iters,
+                  sart_subsets=self.subsets, relax=0.3, positivity=False, fill_seed=self.seed + k)
+        from aretomo3_amd import _lib as L
+        kw["subset_order"] = L.SUBSETS_INTERLEAVED
+        vx = px // 2 * 2
+        e = dict(pin=pin, tilts=tilts, vz=vz, kw=kw, shape=(px, vz, vx),
+                 updates=float(vx) * px * vz * n * (1 + 1 + 2 * self.iters))
+        self.series[size] = e
+        print(f"[bench] serve pool {size} generated in {time.time() - t0:.1f}s", file=sys.stderr,
+              flush=True)
+        return e
 
-    # plain per-series FP32 path: each pool series alone (WBP + SART), timed
-    plain_s = []
-    for k, e in enumerate(pool):
-        n, py, px = e["shape"]
-        d_in = DeviceArray.from_numpy(e["pin"].array)
-        ps = Plan(px, py, e["tilts"], e["vz"], L.METHOD_SART, storage=L.STORAGE_F32, **e["kw"])
-        pw = Plan(px, py, e["tilts"], e["vz"], L.METHOD_WBP, storage=L.STORAGE_F32,
-                  tilt_axis=e["kw"]["tilt_axis"], shifts=e["kw"]["shifts"], fill_seed=e["kw"]["fill_seed"])
-        d_out = DeviceArray(4 * ps.vol_x * ps.vol_y * ps.vol_z)
-        d_w = DeviceArray(4 * pw.vol_x * pw.vol_y * pw.vol_z)
-        ps.execute(d_in, d_out)
-        pw.execute(d_in, d_w)
-        synchronize()
-        t0 = time.perf_counter()
-        h_out = np.empty(ps.volume_shape, np.float32)
-        d_in.upload(e["pin"].array)
-        pw.execute(d_in, d_w)
-        ps.execute(d_in, d_out)
-        synchronize()
-        h_out = d_out.to_numpy(ps.volume_shape)
-        h_w = d_w.to_numpy(pw.volume_shape)
-        plain_s.append(time.perf_counter() - t0)
+    def outputs(self, shape):
+        """Pinned SART and WBP output views of `shape` (one pair, sized for the
+        largest volume; copy-out is serialised, so requests can share it)."""
+        from aretomo3_amd import PinnedArray
+        need = int(np.prod(shape))
+        if self.out is None or self.out[0].array.size < need:
+            cap = max(need, 2048 * 512 * 2048)
+            self.out = (PinnedArray((cap,)), PinnedArray((cap,)))
+        return (self.out[0].array[:need].reshape(shape), self.out[1].array[:need].reshape(shape))
+
+    def plain(self, size):
+        """Seconds of the plain per-series FP32 path for one request of this
+        size (second of two runs)."""
+        if size in self.plain_s:
+            return self.plain_s[size]
+        from aretomo3_amd import DeviceArray, Plan, synchronize
+        from aretomo3_amd import _lib as L
+        e = self.get(size)
+        n, px, vz = size
+        ps = Plan(px, px, e["tilts"], vz, L.METHOD_SART, storage=L.STORAGE_F32, **e["kw"])
+        pw = Plan(px, px, e["tilts"], vz, L.METHOD_WBP, storage=L.STORAGE_F32,
+                  tilt_axis=e["kw"]["tilt_axis"], shifts=e["kw"]["shifts"],
+                  fill_seed=e["kw"]["fill_seed"])
+        d_in = DeviceArray(e["pin"].array.nbytes)
+        nb = 4 * ps.vol_x * ps.vol_y * ps.vol_z
+        d_out, d_w = DeviceArray(nb), DeviceArray(nb)
+        h_out, h_w = self.outputs(ps.volume_shape)
+        times = []
+        for _ in range(2):
+            synchronize()
+            t0 = time.perf_counter()
+            L.check(L.lib().atr_memcpy_h2d(d_in.ptr, e["pin"].array.ctypes.data, e["pin"].array.nbytes))
+            pw.execute(d_in, d_w)
+            ps.execute(d_in, d_out)
+            synchronize()
+            L.check(L.lib().atr_memcpy_d2h(h_out.ctypes.data, d_out.ptr, nb))
+            L.check(L.lib().atr_memcpy_d2h(h_w.ctypes.data, d_w.ptr, nb))
+            times.append(time.perf_counter() - t0)
         ps.close(), pw.close()
-        del d_in, d_out, d_w, h_out, h_w
-    plain_total = sum(plain_s[k] for k in picks)
+        d_in.free(), d_out.free(), d_w.free()
+        self.plain_s[size] = times[-1]
+        return times[-1]
 
-    srv = Server(storage=L.STORAGE_F16)
-    # warm-up: one small request through the server
-    e = pool[1]
-    srv.wait(srv.submit(e["pin"].array, e["tilts"], e["vz"], with_wbp=True, **e["kw"]))
+
+def serve_stream(pool, sizes, rate, seed, max_active):
+    """One open-loop stream: request k arrives after Exp(1/rate) gaps and is
+    submitted to a fresh server (FP16 storage).  Returns the per-request
+    records and the makespan (first arrival to last completion)."""
+    from aretomo3_amd import Server
+    from aretomo3_amd import _lib as L
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(sizes))
+    srv = Server(storage=L.STORAGE_F16, max_active=max_active)
+    # warm-up: one small request through this server
+    e = pool.get((31, 512, 128))
+    out, wout = pool.outputs(e["shape"])
+    srv.wait(srv.submit(e["pin"].array, e["tilts"], e["vz"], with_wbp=True, out=out, wbp_out=wout,
+                        **e["kw"]))
     ids = []
     t_start = time.perf_counter()
     t_next = t_start
-    for k, g in zip(picks, gaps):
+    for size, g in zip(sizes, gaps):
         t_next += g
         now = time.perf_counter()
         if t_next > now:
             time.sleep(t_next - now)
-        e = pool[k]
-        ids.append(srv.submit(e["pin"].array, e["tilts"], e["vz"], with_wbp=True, **e["kw"]))
-    lat = []
-    for rid in ids:
+        e = pool.get(size)
+        out, wout = pool.outputs(e["shape"])
+        ids.append(srv.submit(e["pin"].array, e["tilts"], e["vz"], with_wbp=True, out=out,
+                              wbp_out=wout, **e["kw"]))
+    recs = []
+    for size, rid in zip(sizes, ids):
         _, _, info = srv.wait(rid)
-        lat.append(info["latency_ms"])
-    makespan = time.perf_counter() - t_start
+        recs.append(dict(size=size, **info))
+    t_end = time.perf_counter()
     stats = srv.stats()
     srv.close()
-    lat = np.array(lat)
-    updates = sum(pool[k]["updates"] for k in picks)
-    tps = len(picks) / makespan
-    plain_tps = len(picks) / plain_total
+    first = min(r["submit_ms"] for r in recs)
+    makespan = max(r["done_ms"] for r in recs) - first
+    return recs, makespan / 1e3, stats, t_end - t_start
+
+
+def serve_summary(pool, sizes, recs, makespan, stats, rate):
+    """tomograms/s, p50/p99 latency, its breakdown per request class and the
+    plain per-series FP32 path on the same requests."""
+    lat = np.array([r["latency_ms"] for r in recs])
+    plain_total = sum(pool.plain(s) for s in sizes)
+    tps = len(sizes) / makespan
+    plain_tps = len(sizes) / plain_total
+    updates = sum(pool.get(s)["updates"] for s in sizes)
+
+    def part(sel):
+        if not sel:
+            return None
+        pick = lambda k: np.array([r[k] for r in sel])
+        return {"requests": len(sel),
+                "latency_ms_p50": round(float(np.percentile(pick("latency_ms"), 50)), 1),
+                "latency_ms_p99": round(float(np.percentile(pick("latency_ms"), 99)), 1),
+                "queue_ms_mean": round(float(pick("queue_ms").mean()), 1),
+                "gpu_ms_mean": round(float(pick("gpu_ms").mean()), 1),
+                "copy_out_ms_mean": round(float(pick("copy_out_ms").mean()), 1)}
+    return {
+        "requests": len(sizes),
+        "arrival_rate_per_s": rate,
+        "tomograms_per_s": round(tps, 3),
+        "gvoxel_tilt_updates_per_s": round(updates / makespan / 1e9, 1),
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 1),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 1),
+        "makespan_s": round(makespan, 3),
+        "rounds": stats["rounds"],
+        "max_series_per_round": stats["max_batch"],
+        "large_requests": int(sum(1 for s in sizes if s[1] == 2048)),
+        "by_class": {"large_2048": part([r for r in recs if r["size"][1] == 2048]),
+                     "batching_sizes": part([r for r in recs if r["size"][1] != 2048])},
+        "plain_fp32_per_series_tomograms_per_s": round(plain_tps, 3),
+        "speedup_vs_plain": round(tps / plain_tps, 3),
+    }
+
+
+SERVE_WORKLOAD = ("serve: open-loop Poisson stream, WBP + SART %d iterations x %d interleaved "
+                  "subsets (relax 0.3, positivity off) per request, tilts {31,41,51,61} over +-60 deg, "
+                  "projections {512,768,1024}^2, thickness {128,256,384}, tilt axis 80-95 deg + "
+                  "shifts +-3 px, ~10%% large (2048^2 x 61 x 512), FP16 projection storage, FP32 "
+                  "accumulation, admission by the device-memory budget (288 GB card), pinned host "
+                  "buffers; plain = the same requests one after another through the per-series "
+                  "FP32 path (upload, WBP, SART, copy-back)")
+
+
+def serve_bench(args):
+    """The serving configuration at its stated size: 500 requests per arrival
+    rate, rates swept over 1-20/s (--rate picks one), the same request sizes at
+    every rate.  Prints one JSON line with a serving block per rate."""
+    c = dict(SERVE)
+    n_req = args.requests or c["requests"]
+    rates = [args.rate] if args.rate else [float(r) for r in c["rates"].split(",")]
+    pool = ServePool(c["seed"], c["iters"], c["subsets"])
+    sizes = serve_sizes(np.random.default_rng(c["seed"]), n_req, c["large_frac"])
+    for sz in sorted(set(sizes)):
+        pool.get(sz)
+        pool.plain(sz)
+    per_rate = []
+    for rate in rates:
+        recs, makespan, stats, wall = serve_stream(pool, sizes, rate, c["seed"] + int(rate * 100),
+                                                   args.max_active or c["max_active"])
+        summ = serve_summary(pool, sizes, recs, makespan, stats, rate)
+        per_rate.append(summ)
+        print(f"[bench] serve rate {rate}/s: {json.dumps(summ)}", file=sys.stderr, flush=True)
+    top = max(per_rate, key=lambda r: r["tomograms_per_s"])
+    updates = sum(pool.get(s)["updates"] for s in sizes)
     result = {
         "metric": METRIC,
-        "value": round(updates / makespan / 1e9, 2),
+        "value": top["gvoxel_tilt_updates_per_s"],
         "unit": "Gvoxel-tilt updates/s",
         "n_gpus": 1,
-        "steps": len(picks),
+        "steps": n_req,
         "warmup": 1,
-        "ms_per_step": round(1e3 * makespan / len(picks), 3),
+        "ms_per_step": round(1e3 * top["makespan_s"] / n_req, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded ellipsoid phantoms, analytic projections)",
-        "config": {
-            "workload": ("serve: %d requests, Poisson arrivals at %g/s, WBP + SART %d iterations x %d "
-                         "interleaved subsets (relax 0.3, positivity off) per request, tilts "
-                         "{31,41,51,61} over +-60 deg, projections {512,768,1024}^2, thickness "
-                         "{128,256,384}, tilt axis 80-95 deg + shifts +-3 px, ~%d%% large "
-                         "(2048^2 x 61 x 512), FP16 projection storage, FP32 accumulation; "
-                         "requests drawn from %d distinct seeded series" %
-                         (len(picks), c["rate"], c["iters"], c["subsets"], int(100 * c["large_frac"]),
-                          c["pool"])),
-            "projection_storage": "f16",
-        },
-        "serving": {
-            "tomograms_per_s": round(tps, 3),
-            "latency_ms_p50": round(float(np.percentile(lat, 50)), 1),
-            "latency_ms_p99": round(float(np.percentile(lat, 99)), 1),
-            "makespan_s": round(makespan, 3),
-            "rounds": stats["rounds"],
-            "max_series_per_round": stats["max_batch"],
-            "plain_fp32_per_series_tomograms_per_s": round(plain_tps, 3),
-            "speedup_vs_plain": round(tps / plain_tps, 3),
-            "large_requests": int(sum(1 for k in picks if k == 0)),
-        },
+        "config": {"workload": SERVE_WORKLOAD % (c["iters"], c["subsets"]) +
+                   "; %d requests per rate, rates %s/s" % (n_req, ",".join("%g" % r for r in rates)),
+                   "updates_per_stream": updates, "projection_storage": "f16",
+                   "max_active": args.max_active or c["max_active"]},
+        "serving": per_rate,
     }
     print(json.dumps(result))
+
+
+def serve_block_default():
+    """The serving block of the default benchmark line: a short C5-shaped
+    stream (SERVE_DEFAULT requests) at a rate the GPU sustains, for p50/p99
+    latency, and the same requests as a burst, for throughput against the
+    plain per-series FP32 path."""
+    c, d = dict(SERVE), dict(SERVE_DEFAULT)
+    pool = ServePool(c["seed"], c["iters"], c["subsets"])
+    sizes = serve_sizes(np.random.default_rng(c["seed"] + 7), d["requests"], c["large_frac"])
+    for sz in sorted(set(sizes)):
+        pool.get(sz)
+        pool.plain(sz)
+    out = {}
+    for name, rate in (("steady", d["rate"]), ("burst", d["burst_rate"])):
+        recs, makespan, stats, _ = serve_stream(pool, sizes, rate, c["seed"] + 11, c["max_active"])
+        out[name] = serve_summary(pool, sizes, recs, makespan, stats, rate)
+        print(f"[bench] serve {name}: {json.dumps(out[name])}", file=sys.stderr, flush=True)
+    return {"workload": SERVE_WORKLOAD % (c["iters"], c["subsets"]) +
+            "; %d requests, steady at %g/s and burst at %g/s" % (d["requests"], d["rate"],
+                                                                   d["burst_rate"]),
+            "tomograms_per_s": out["burst"]["tomograms_per_s"],
+            "latency_ms_p50": out["steady"]["latency_ms_p50"],
+            "latency_ms_p99": out["steady"]["latency_ms_p99"],
+            "speedup_vs_plain": out["burst"]["speedup_vs_plain"],
+            "plain_fp32_per_series_tomograms_per_s": out["burst"]["plain_fp32_per_series_tomograms_per_s"],
+            "steady": out["steady"], "burst": out["burst"]}
 
 
 BATCH = dict(series=16, seed=20261205, iters=15, subsets=5)
@@ -308,8 +463,7 @@ def batch_bench(args):
         ps = Plan(px, px, tilts, vz, L.METHOD_SART, sart_iters=c["iters"], sart_subsets=c["subsets"],
                   relax=0.3, subset_order=L.SUBSETS_INTERLEAVED, positivity=False, **geo)
         if not args.no_tune:
-            tz, ch, rows = ps.autotune()
-            ps.set_tile_shape(16, 16, rows)   # a batch shares the default back-projection shape
+            ps.autotune()   # each batch entry runs its own series' tile shapes
         pw = Plan(px, px, tilts, vz, L.METHOD_WBP, **geo)
         sart_plans.append(ps)
         wbp_plans.append(pw)
@@ -570,6 +724,9 @@ def main():
     ap.add_argument("--config", default="sart1024", choices=sorted(CONFIGS) + ["serve", "batch16", "extras"])
     ap.add_argument("--requests", type=int, default=0, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second")
+    ap.add_argument("--max-active", type=int, default=0, help="serve: series per round at most")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="default run: leave out the serving block")
     ap.add_argument("--small", action="store_true", help="same as --config small")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
@@ -702,6 +859,7 @@ def main():
             kernels[kname]["flop_model"] = {"flop_per_launch": flop, "tflops": round(tf, 2),
                                             "frac_of_fp32_vector_peak_157.3": round(tf / 157.3, 4)}
     traffic = measured_traffic() if name == "sart1024" else None
+    copy_peak = measured_copy_peak()
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -720,7 +878,8 @@ def main():
             "updates_per_step": updates,
             "projection_storage": w["storage"],
             "tile_shape": None if tuned is None else {"bp_z_rows": tuned[0], "bp_tilts_per_stage": tuned[1],
-                                                      "fp_band_rows": tuned[2], "source": "autotuner"},
+                                                      "fp_band_rows": tuned[2],
+                                                      "fp_band_prefetch": tuned[3], "source": "autotuner"},
         },
         "roofline": {
             "kernel": rkernel,
@@ -734,6 +893,7 @@ def main():
                                "launch, copy-kernel calibrated)") if traffic else None,
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": round(avg_ms, 4),
+            "measured_copy_peak": copy_peak,
         },
         "kernels": kernels,
     }
@@ -763,6 +923,7 @@ def main():
             "value": round(cpu_updates / t_cpu / 1e9, 4),
             "unit": "Gvoxel-tilt updates/s",
             "cores": threads,
+            "nproc": len(os.sched_getaffinity(0)),
             "kind": "port",
             "sample": "FP64 OpenMP oracle, %d of %d y-slices of the same series (%s only; "
                       "pre-stage of all tilts took %.1fs, not counted), %.1fs" %
@@ -773,6 +934,13 @@ def main():
         result["frc_vs_fp64"] = {"min_to_nyquist": round(float(frc.min()), 6),
                                  "at_half_nyquist": round(float(frc[len(frc) // 2]), 6),
                                  "rings": len(frc)}
+    if name == "sart1024" and not args.no_serve:
+        # the latency half of the metric: a short serving stream of the
+        # serving configuration's request mix, measured in the same run
+        plan.close()
+        d_raw.free()
+        d_vol.free()
+        result["serving"] = serve_block_default()
     print(json.dumps(result))
 
 

@@ -159,20 +159,25 @@ atr_status atr_execute_host(atr_plan* plan, const float* proj_host, float* vol_h
  * launch constants, Recon/GBackProj.cu:79-82, Recon/GForProj.cu:106-110):
  * times the candidate tile shapes of a SART plan on its own geometry -- back-
  * projection z rows per thread {16, 8} x tilts per LDS stage {16, 8}, then
- * forward-projection band rows -- and keeps the fastest.  Choices are cached
- * per shape within the process.  Every voxel sums its tilts in the same
- * order whatever the shape; the FP32 rounding of the back-projection
- * coordinate (taken relative to the tile's window) and the order in which a
- * forward ray adds up its z-bands depend on the shape, at the 1e-7 level.
- * Outputs may be NULL.  A candidate whose launch fails is dropped, never
- * cached.  A plan that belongs to a batch (atr_batch_create until
- * atr_batch_destroy) cannot be retuned: ATR_ERR_INVALID_ARG. */
+ * forward-projection band rows, for the direct-staging forward projector and
+ * (VolX a multiple of 256) the band-prefetching one (fp_prefetch = 1: one
+ * wave streams the next z-band into LDS while the others sample) -- and keeps
+ * the fastest.  Choices are cached per shape within the process.  Every voxel
+ * sums its tilts in the same order whatever the shape; the FP32 rounding of
+ * the back-projection coordinate (taken relative to the tile's window) and
+ * the order in which a forward ray adds up its z-bands depend on the shape,
+ * at the 1e-7 level (the two forward projectors give the same result for
+ * the same band rows).  Outputs may be NULL.  A candidate whose launch fails
+ * is dropped, never cached.  A plan that belongs to a batch (atr_batch_create
+ * until atr_batch_destroy) cannot be retuned: ATR_ERR_INVALID_ARG. */
 atr_status atr_plan_autotune(atr_plan* plan, void* stream, int* bp_tz, int* bp_chunk,
-                             int* fp_band_rows);
-/* Sets the tile shapes by hand: bp_tz, bp_chunk in {8, 16}; fp_band_rows in
- * [1, the largest band that fits in LDS].  Refused for a plan that belongs
- * to a batch, as atr_plan_autotune. */
-atr_status atr_plan_set_tile_shape(atr_plan* plan, int bp_tz, int bp_chunk, int fp_band_rows);
+                             int* fp_band_rows, int* fp_prefetch);
+/* Sets the tile shapes by hand: bp_tz, bp_chunk in {8, 16}; fp_prefetch 0 or
+ * 1 (1 needs VolX a multiple of 256); fp_band_rows in [1, the largest band
+ * that fits in LDS for that forward projector].  Refused for a plan that
+ * belongs to a batch, as atr_plan_autotune. */
+atr_status atr_plan_set_tile_shape(atr_plan* plan, int bp_tz, int bp_chunk, int fp_band_rows,
+                                   int fp_prefetch);
 /* The forward-projection band heights (cell rows) the autotuner times for a
  * slice of vol_x x vol_z, without a GPU: every one fits the LDS, none
  * appears twice.  *n receives their number (at most 3); rows may be NULL. */
@@ -233,7 +238,10 @@ void atr_batch_destroy(atr_batch* batch);
  * pre-stage to output, and advances all admitted series together: every
  * round is one persistent forward-projection and one persistent
  * back-projection launch carrying one SART step of each series.  Requests
- * admitted mid-run join at the next round.  Each result equals what the
+ * admitted mid-run join at the next round.  Plans are built on a host thread
+ * of the server ahead of admission; input uploads, WBP reconstructions and
+ * copy-out run on their own streams beside the rounds; device memory comes
+ * from an arena held until atr_server_destroy.  Each result equals what the
  * series gives alone. */
 typedef struct atr_server atr_server;
 typedef struct atr_server_config {
@@ -241,10 +249,16 @@ typedef struct atr_server_config {
     int max_active;          /* series reconstructed together at most (default 4), 0 = no limit */
     int storage;             /* projection storage of every request (default F16) */
 } atr_server_config;
+/* Times on the host clock, ms since the server started.  Per request:
+ * queue = admit - submit (waiting for budget or a slot), GPU = compute_done -
+ * admit (input upload, WBP and every SART round), copy-out = done -
+ * compute_done (the volumes to host memory). */
 typedef struct atr_request_info {
-    double submit_ms, admit_ms, done_ms; /* host clock, ms since the server started */
+    double submit_ms, admit_ms, done_ms;
     double latency_ms;                   /* done - submit */
     int status;
+    double prepared_ms;                  /* host tables of its plans built */
+    double compute_done_ms;              /* its last GPU kernel finished */
 } atr_request_info;
 void atr_server_config_init(atr_server_config* cfg);
 atr_status atr_server_create(const atr_server_config* cfg, atr_server** server);
