@@ -40,7 +40,7 @@ EXPORTS = [
     "atr_aligned_size", "atr_mrc_read_info", "atr_mrc_read_sections", "atr_mrc_write", "atr_volume_suffix",
     "atr_aln_read", "atr_aln_get_info", "atr_aln_get_global", "atr_aln_get_local",
     "atr_aln_get_dark", "atr_aln_destroy", "atr_aln_write", "atr_tilt_file_read",
-    "atr_reproj_create", "atr_reproj_execute", "atr_reproj_find_range", "atr_reproj_destroy",
+    "atr_reproj_create", "atr_reproj_execute", "atr_reproj_execute_noalloc", "atr_reproj_find_range", "atr_reproj_destroy",
     "atr_calc_vol_thick", "atr_thickness_edges",
 ]
 BATCH_GRAPH = 1
@@ -200,6 +200,7 @@ def lib():
         "atr_tilt_file_read": (i, [ctypes.c_char_p, i, vp, vp, vp, pi]),
         "atr_reproj_create": (i, [i, i, i, i, ctypes.POINTER(vp)]),
         "atr_reproj_execute": (i, [vp, vp, vp, vp, i, vp, vp]),
+        "atr_reproj_execute_noalloc": (i, [i, i, i, i, vp, vp, vp, i, vp, vp]),
         "atr_reproj_find_range": (i, [vp, vp, i, i, pi, pi]),
         "atr_reproj_destroy": (None, [vp]),
         "atr_calc_vol_thick": (i, [ctypes.POINTER(SeriesDesc), f, vp, ctypes.POINTER(Thickness), vp]),

@@ -317,6 +317,19 @@ atr_status atr_reproj_create(int proj_x, int proj_y, int n_tilts, int vol_z, atr
 atr_status atr_reproj_execute(atr_reproj* reproj, const float* proj_dev, const float* tilts,
                               const unsigned char* skip, int proj_idx, float* reproj_dev,
                               void* stream);
+/* No counterpart in the reference: atr_reproj_execute without a handle, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It allocates
+ * nothing: the back-projected slab lives in a fixed 64 MiB device array of
+ * the library (one per GPU, held until the process ends), so one slab row,
+ * 2 proj_x x vol_z floats, must fit 64 MiB (2 proj_x vol_z <= 16777216),
+ * else ATR_ERR_INVALID_ARG.  Same kernels and results as
+ * atr_reproj_execute.  Every call on one GPU uses the same array: calls that
+ * overlap on one GPU (other streams or threads) race. */
+atr_status atr_reproj_execute_noalloc(int proj_x, int proj_y, int n_tilts, int vol_z,
+                                      const float* proj_dev, const float* tilts,
+                                      const unsigned char* skip, int proj_idx, float* reproj_dev,
+                                      void* stream);
 /* The tilt range of CCalcReproj::mFindProjRange (CCalcReproj.cpp:73-124). */
 atr_status atr_reproj_find_range(const float* tilts, const unsigned char* skip, int n_tilts,
                                  int proj_idx, int* first, int* last);
