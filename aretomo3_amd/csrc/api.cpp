@@ -280,11 +280,17 @@ static atr_status sart_tables_host(Plan& p, int tz, int chunk, int bz, int pf)
     sched.clear();
     p.subset_pair_off.clear();
     const int waves = fp_waves(p.fp_pf);
-    for (int sb = 0; sb < p.n_subsets; sb++) {
-        p.subset_pair_off.push_back((int)sched.size());
+    // the subsets' schedules are independent: built on host threads, then
+    // laid out in subset order
+    std::vector<std::vector<int>> part(p.n_subsets);
+    parallel_for(p.n_subsets, [&](int sb) {
         const int f0 = p.subset_fp_off[sb], f1 = p.subset_fp_off[sb + 1];
         fp_pair_schedule(g, p.subset_fp.data() + f0, f1 - f0, p.proj,
-                         fp_group(p.proj, f1 - f0, waves), sched, waves);
+                         fp_group(p.proj, f1 - f0, waves), part[sb], waves);
+    });
+    for (int sb = 0; sb < p.n_subsets; sb++) {
+        p.subset_pair_off.push_back((int)sched.size());
+        sched.insert(sched.end(), part[sb].begin(), part[sb].end());
     }
     return ATR_OK;
 }

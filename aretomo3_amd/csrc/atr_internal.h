@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/aretomo_recon.h"
@@ -324,6 +325,8 @@ int fp_band_rows(int vol_x, int pf = 0);
 int fp_balanced_rows(int bz_max, int vol_z);
 void fp_band_candidates(int vol_x, int vol_z, std::vector<int>& rows, int pf = 0);
 int fp_group(int proj, int n_list, int n_waves = FP_WAVES);
+// fn(0) .. fn(n - 1) on up to 16 host threads (returns when all are done).
+void parallel_for(int n, const std::function<void(int)>& fn);
 void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, int group,
                       std::vector<int>& tab, int n_waves = FP_WAVES);
 void split_items(int n_items, int n_splits, std::vector<int>& starts, std::vector<int>& sizes);

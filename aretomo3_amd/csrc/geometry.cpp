@@ -13,6 +13,7 @@
 // -ffp-contract=off, so every inclusion decision is reproducible bit for bit.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <thread>
 
 #include "atr_internal.h"
@@ -104,6 +105,8 @@ static void parallel_tilts(int n, F fn)
         });
     for (auto& x : th) x.join();
 }
+
+void parallel_for(int n, const std::function<void(int)>& fn) { parallel_tilts(n, fn); }
 
 // Recon/GBackProj.cu:25-37: u = fx cos + fz sin + ProjX/2, used iff
 // 0 <= u <= ProjX-2, fx = x - VolX*0.5, fz = z - VolZ*0.5.
@@ -465,34 +468,68 @@ void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, 
                 bmax[b] = std::max(bmax[b], load[(size_t)best * nb + b]);
             }
         }
-        // pairwise exchanges between waves while they lower the objective
-        auto objective = [&]() {
-            double o = 0.0;
-            for (int b = 0; b < nb; b++) o += band_max(b);
-            return o;
+        // pairwise exchanges between waves while they lower the objective.
+        // An exchange between waves u and v changes only their loads, so a
+        // candidate is scored from the three largest loads of each band
+        // (the largest one not on u or v stands for every other wave): O(nb)
+        // per candidate instead of O(nb * n_waves).
+        double cur = 0.0;
+        for (int b = 0; b < nb; b++) cur += band_max(b);
+        std::vector<double> top_v((size_t)nb * 3);
+        std::vector<int> top_w((size_t)nb * 3);
+        auto rank_band = [&](int b) {
+            double* tv = &top_v[(size_t)b * 3];
+            int* tw = &top_w[(size_t)b * 3];
+            for (int k = 0; k < 3; k++) {
+                tv[k] = 0.0;
+                tw[k] = -1;
+            }
+            for (int w = 0; w < n_waves; w++) {
+                const double l = load[(size_t)w * nb + b];
+                for (int k = 0; k < 3; k++) {
+                    if (tw[k] < 0 || l > tv[k]) {
+                        for (int m = 2; m > k; m--) {
+                            tv[m] = tv[m - 1];
+                            tw[m] = tw[m - 1];
+                        }
+                        tv[k] = l;
+                        tw[k] = w;
+                        break;
+                    }
+                }
+            }
         };
-        double cur = objective();
+        for (int b = 0; b < nb; b++) rank_band(b);
         for (int pass = 0; pass < 4; pass++) {
             bool improved = false;
             for (int a = 0; a < n_items; a++)
                 for (int c = a + 1; c < n_items; c++) {
                     const int u = owner[a], v = owner[c];
                     if (u == v) continue;
+                    double o = 0.0;
                     for (int b = 0; b < nb; b++) {
                         const double d = cost[(size_t)c * nb + b] - cost[(size_t)a * nb + b];
-                        load[(size_t)u * nb + b] += d;
-                        load[(size_t)v * nb + b] -= d;
+                        const double lu = load[(size_t)u * nb + b] + d;
+                        const double lv = load[(size_t)v * nb + b] - d;
+                        double m = std::max(lu, lv);
+                        for (int k = 0; k < 3; k++) {
+                            const int w = top_w[(size_t)b * 3 + k];
+                            if (w >= 0 && w != u && w != v) {
+                                m = std::max(m, top_v[(size_t)b * 3 + k]);
+                                break;
+                            }
+                        }
+                        o += m;
                     }
-                    const double o = objective();
                     if (o < cur - 1e-9) {
                         cur = o;
                         std::swap(owner[a], owner[c]);
                         improved = true;
-                    } else {
                         for (int b = 0; b < nb; b++) {
                             const double d = cost[(size_t)c * nb + b] - cost[(size_t)a * nb + b];
-                            load[(size_t)u * nb + b] -= d;
-                            load[(size_t)v * nb + b] += d;
+                            load[(size_t)u * nb + b] += d;
+                            load[(size_t)v * nb + b] -= d;
+                            rank_band(b);
                         }
                     }
                 }
