@@ -44,6 +44,7 @@ EXPORTS = [
     "atr_calc_vol_thick", "atr_thickness_edges",
 ]
 BATCH_GRAPH = 1
+SCHEDULE_FAIR_SHARE, SCHEDULE_EVERY_STEP = 0, 1
 
 
 class SeriesDesc(ctypes.Structure):
@@ -80,7 +81,7 @@ class SeriesDesc(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     """Mirror of atr_server_config."""
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_active", ctypes.c_int),
-                ("storage", ctypes.c_int)]
+                ("storage", ctypes.c_int), ("schedule", ctypes.c_int)]
 
 
 class RequestInfo(ctypes.Structure):

@@ -15,9 +15,9 @@
 //     round, a finished one leaves);
 //   * the finisher copies finished volumes out and returns their memory.
 // Device work runs on four streams: rounds (compute), WBP reconstructions
-// (compute), host-to-device and device-to-host copies, ordered by events, so
-// a request's input upload, its WBP and the copy-out of earlier volumes
-// overlap the rounds.  Device memory comes from an arena that keeps what it
+// (compute, each followed by the copy-out of its volume), host-to-device and
+// device-to-host copies, ordered by events, so a request's input upload, its
+// WBP and the copy-out of earlier volumes overlap the rounds.  Device memory comes from an arena that keeps what it
 // took from HIP until the server is destroyed: admitting and releasing a
 // request synchronises nothing.  A series computes exactly what it computes
 // alone: the same kernels, the same operation order, nothing shared across
@@ -154,6 +154,8 @@ struct Request {
     float* d_wbp = nullptr;
     hipEvent_t ev_in = nullptr, ev_sart = nullptr, ev_wbp = nullptr, ev_done = nullptr;
     int step = 0, last = 0;  // SART steps: 0 initial BP, 1..last iterations
+    double cost = 0.0;       // voxel-tilt updates of one SART step (fair-share rounds)
+    double credit = 0.0;     // fair-share credit, in the same unit
 };
 
 }  // namespace
@@ -305,8 +307,12 @@ static atr_status admit(atr_server* s, Request& r)
     ATR_HIP(hipMemcpyAsync(r.d_in, r.proj_host, r.in_bytes, hipMemcpyHostToDevice, s->s_h2d));
     ATR_HIP(hipEventRecord(r.ev_in, s->s_h2d));
     if (r.wbp) {
+        // the WBP volume is final as soon as its kernels end: it is copied
+        // out right behind them on the same stream, while the SART rounds
+        // of the series still run (ev_wbp: WBP volume on the host)
         ATR_HIP(hipStreamWaitEvent(s->s_wbp, r.ev_in, 0));
         if ((st = plan_execute(*r.wbp, r.d_in, r.d_wbp, 0, s->s_wbp)) != ATR_OK) return st;
+        ATR_HIP(hipMemcpyAsync(r.wbp_host, r.d_wbp, r.wbp_bytes, hipMemcpyDeviceToHost, s->s_wbp));
         ATR_HIP(hipEventRecord(r.ev_wbp, s->s_wbp));
     }
     hipError_t e = hipGetLastError();
@@ -323,6 +329,8 @@ static atr_status start_sart(atr_server* s, Request& r)
     sart_prepare(p, r.d_in, r.d_out, 0, s->s_comp);
     r.step = p.all_tilts.empty() ? 1 : 0;
     r.last = p.iters * p.n_subsets;
+    r.cost = (double)p.vol_x * p.vol_y * p.vol_z * ((double)p.n / std::max(1, p.n_subsets));
+    r.credit = 0.0;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "serving start");
     return ATR_OK;
@@ -354,6 +362,15 @@ static atr_status grow_slots(atr_server* s, int need)
 
 // One round: one SART step of every admitted series.  A series past its last
 // step gets its fill + layout on the round stream and goes to the finisher.
+//
+// Which series take a step (fair share, the default): deficit round robin
+// on the voxel-tilt updates of a step.  Every round each series earns the
+// step cost of the cheapest one and takes a step when its credit covers its
+// own cost, so the cheapest series steps every round and one whose steps
+// cost m times as much every m-th round: every series gets the same GPU
+// work per round, and a small series admitted beside a large one finishes in
+// about its own share of the time instead of at the large one's step rate.
+// ATR_SCHEDULE_EVERY_STEP: every series takes one step every round.
 static atr_status run_round(atr_server* s)
 {
     auto& act = s->active;
@@ -362,10 +379,23 @@ static atr_status run_round(atr_server* s)
     const int k = s->slot;
     s->slot = (s->slot + 1) % kRing;
     if (s->slot_used[k]) ATR_HIP(hipEventSynchronize(s->ev_slot[k]));
-    int n_fp = 0, n_bp = 0, tot_fp = 0, tot_bp = 0;
+    std::vector<char> take(act.size(), 1);
+    if (s->cfg.schedule == ATR_SCHEDULE_FAIR_SHARE) {
+        double quantum = 0.0;
+        for (auto& rp : act) quantum = quantum > 0.0 ? std::min(quantum, rp->cost) : rp->cost;
+        for (size_t i = 0; i < act.size(); i++) {
+            Request& r = *act[i];
+            r.credit += quantum;
+            take[i] = r.credit >= r.cost * (1.0 - 1e-9);
+            if (take[i]) r.credit -= r.cost;
+        }
+    }
+    int n_fp = 0, n_bp = 0, tot_fp = 0, tot_bp = 0, n_take = 0;
     size_t shm = 0;
-    for (auto& rp : act) {
-        Request& r = *rp;
+    for (size_t i = 0; i < act.size(); i++) {
+        if (!take[i]) continue;
+        n_take++;
+        Request& r = *act[i];
         const Plan& p = *r.sart;
         const int n_all = (int)p.all_tilts.size();
         if (r.step == 0) {
@@ -416,10 +446,15 @@ static atr_status run_round(atr_server* s)
     ATR_HIP(hipEventRecord(s->ev_slot[k], q));
     s->slot_used[k] = true;
     s->rounds++;
-    s->max_batch = std::max<uint64_t>(s->max_batch, act.size());
+    s->max_batch = std::max<uint64_t>(s->max_batch, (uint64_t)n_take);
     std::vector<std::shared_ptr<Request>> keep;
-    for (auto& rp : act) {
+    for (size_t i = 0; i < act.size(); i++) {
+        auto& rp = act[i];
         Request& r = *rp;
+        if (!take[i]) {
+            keep.push_back(rp);
+            continue;
+        }
         r.step++;
         if (r.step > r.last) {
             sart_finish(*r.sart, r.d_out, q);
@@ -579,14 +614,13 @@ static void fin_main(atr_server* s)
         }
         Request& r = *rp;
         hipError_t e = hipEventSynchronize(r.ev_sart);
-        if (e == hipSuccess && r.wbp) e = hipEventSynchronize(r.ev_wbp);
         r.compute_done = Clock::now();
         if (e == hipSuccess)
             e = hipMemcpyAsync(r.vol_host, r.d_out, r.out_bytes, hipMemcpyDeviceToHost, s->s_d2h);
-        if (e == hipSuccess && r.wbp)
-            e = hipMemcpyAsync(r.wbp_host, r.d_wbp, r.wbp_bytes, hipMemcpyDeviceToHost, s->s_d2h);
         if (e == hipSuccess) e = hipEventRecord(r.ev_done, s->s_d2h);
         if (e == hipSuccess) e = hipEventSynchronize(r.ev_done);
+        // the WBP volume went out behind its kernels (usually long done)
+        if (e == hipSuccess && r.wbp) e = hipEventSynchronize(r.ev_wbp);
         if (e != hipSuccess) {
             (void)hipStreamSynchronize(s->s_d2h);
             r.status = ATR_ERR_HIP;
@@ -607,7 +641,8 @@ void atr_server_config_init(atr_server_config* c)
 {
     if (!c) return;
     std::memset(c, 0, sizeof(*c));
-    c->max_active = 4;   // a few series overlap; later arrivals queue (lower p50 than all-at-once)
+    c->max_active = 8;   // several series overlap; later arrivals queue (lower p50 than all-at-once)
+    c->schedule = ATR_SCHEDULE_FAIR_SHARE;
     c->storage = ATR_STORAGE_F16;
 }
 
@@ -635,6 +670,8 @@ atr_status atr_server_create(const atr_server_config* cfg, atr_server** out)
     if (!cfg || !out) return fail(ATR_ERR_INVALID_ARG, "atr_server_create: NULL argument");
     if (cfg->storage < ATR_STORAGE_F32 || cfg->storage > ATR_STORAGE_BF16)
         return fail(ATR_ERR_INVALID_ARG, "atr_server_create: unknown storage");
+    if (cfg->schedule != ATR_SCHEDULE_FAIR_SHARE && cfg->schedule != ATR_SCHEDULE_EVERY_STEP)
+        return fail(ATR_ERR_INVALID_ARG, "atr_server_create: unknown schedule");
     *out = nullptr;
     auto* s = new atr_server;
     s->cfg = *cfg;

@@ -290,10 +290,12 @@ class Server:
     WBP) as they arrive; each is admitted by the HBM budget and reconstructed
     together with the others on the GPU.  Output arrays are allocated here."""
 
-    def __init__(self, hbm_budget=0, max_active=None, storage=L.STORAGE_F16):
+    def __init__(self, hbm_budget=0, max_active=None, storage=L.STORAGE_F16,
+                 schedule=L.SCHEDULE_FAIR_SHARE):
         cfg = L.ServerConfig()
         lib().atr_server_config_init(ctypes.byref(cfg))
         cfg.hbm_budget_bytes, cfg.storage = int(hbm_budget), int(storage)
+        cfg.schedule = int(schedule)
         if max_active is not None:
             cfg.max_active = int(max_active)
         h = ctypes.c_void_p()

@@ -171,8 +171,9 @@ def describe(name, w, vol):
              alg, w["storage"].upper()))
 
 
+SCHEDULES = {"fair": 0, "every": 1}   # ATR_SCHEDULE_FAIR_SHARE / _EVERY_STEP
 SERVE = dict(requests=500, rates="1,2,5,10,20", seed=20261206, large_frac=0.1, iters=10, subsets=5,
-             max_active=4)
+             max_active=8)
 # the short serving stream of the default run: a C5-shaped stream sized to
 # keep the whole default command within a few minutes
 SERVE_DEFAULT = dict(requests=40, rate=4.0, burst_rate=20.0)
@@ -244,39 +245,42 @@ class ServePool:
 
     def plain(self, size):
         """Seconds of the plain per-series FP32 path for one request of this
-        size (second of two runs)."""
+        size (second of two runs): the two plans built (host tables and
+        device memory, as the serving path builds them per request), upload,
+        WBP, SART, both volumes copied back."""
         if size in self.plain_s:
             return self.plain_s[size]
         from aretomo3_amd import DeviceArray, Plan, synchronize
         from aretomo3_amd import _lib as L
         e = self.get(size)
         n, px, vz = size
-        ps = Plan(px, px, e["tilts"], vz, L.METHOD_SART, storage=L.STORAGE_F32, **e["kw"])
-        pw = Plan(px, px, e["tilts"], vz, L.METHOD_WBP, storage=L.STORAGE_F32,
-                  tilt_axis=e["kw"]["tilt_axis"], shifts=e["kw"]["shifts"],
-                  fill_seed=e["kw"]["fill_seed"])
         d_in = DeviceArray(e["pin"].array.nbytes)
-        nb = 4 * ps.vol_x * ps.vol_y * ps.vol_z
+        vx = px // 2 * 2
+        nb = 4 * vx * px * vz
         d_out, d_w = DeviceArray(nb), DeviceArray(nb)
-        h_out, h_w = self.outputs(ps.volume_shape)
         times = []
         for _ in range(2):
             synchronize()
             t0 = time.perf_counter()
+            ps = Plan(px, px, e["tilts"], vz, L.METHOD_SART, storage=L.STORAGE_F32, **e["kw"])
+            pw = Plan(px, px, e["tilts"], vz, L.METHOD_WBP, storage=L.STORAGE_F32,
+                      tilt_axis=e["kw"]["tilt_axis"], shifts=e["kw"]["shifts"],
+                      fill_seed=e["kw"]["fill_seed"])
+            h_out, h_w = self.outputs(ps.volume_shape)
             L.check(L.lib().atr_memcpy_h2d(d_in.ptr, e["pin"].array.ctypes.data, e["pin"].array.nbytes))
             pw.execute(d_in, d_w)
             ps.execute(d_in, d_out)
             synchronize()
             L.check(L.lib().atr_memcpy_d2h(h_out.ctypes.data, d_out.ptr, nb))
             L.check(L.lib().atr_memcpy_d2h(h_w.ctypes.data, d_w.ptr, nb))
+            ps.close(), pw.close()
             times.append(time.perf_counter() - t0)
-        ps.close(), pw.close()
         d_in.free(), d_out.free(), d_w.free()
         self.plain_s[size] = times[-1]
         return times[-1]
 
 
-def serve_stream(pool, sizes, rate, seed, max_active):
+def serve_stream(pool, sizes, rate, seed, max_active, schedule=0):
     """One open-loop stream: request k arrives after Exp(1/rate) gaps and is
     submitted to a fresh server (FP16 storage).  Returns the per-request
     records and the makespan (first arrival to last completion)."""
@@ -284,7 +288,7 @@ def serve_stream(pool, sizes, rate, seed, max_active):
     from aretomo3_amd import _lib as L
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, len(sizes))
-    srv = Server(storage=L.STORAGE_F16, max_active=max_active)
+    srv = Server(storage=L.STORAGE_F16, max_active=max_active, schedule=schedule)
     # warm-up: one small request through this server
     e = pool.get((31, 512, 128))
     out, wout = pool.outputs(e["shape"])
@@ -293,7 +297,7 @@ def serve_stream(pool, sizes, rate, seed, max_active):
     ids = []
     t_start = time.perf_counter()
     t_next = t_start
-    for size, g in zip(sizes, gaps):
+    for k, (size, g) in enumerate(zip(sizes, gaps)):
         t_next += g
         now = time.perf_counter()
         if t_next > now:
@@ -302,6 +306,9 @@ def serve_stream(pool, sizes, rate, seed, max_active):
         out, wout = pool.outputs(e["shape"])
         ids.append(srv.submit(e["pin"].array, e["tilts"], e["vz"], with_wbp=True, out=out,
                               wbp_out=wout, **e["kw"]))
+        if (k + 1) % 25 == 0:
+            print(f"[bench] serve rate {rate}/s: {k + 1} of {len(sizes)} submitted after "
+                  f"{time.perf_counter() - t_start:.1f}s", file=sys.stderr, flush=True)
     recs = []
     for size, rid in zip(sizes, ids):
         _, _, info = srv.wait(rid)
@@ -314,13 +321,29 @@ def serve_stream(pool, sizes, rate, seed, max_active):
     return recs, makespan / 1e3, stats, t_end - t_start
 
 
+def plain_fifo(pool, sizes, recs):
+    """The plain per-series FP32 path on the same arrivals: one series at a
+    time in arrival order, each taking its measured time (upload, WBP, SART,
+    copy-back).  Returns (tomograms/s from first arrival to last completion,
+    per-request latencies in ms)."""
+    arrive = [r["submit_ms"] for r in recs]
+    free_at, lat = arrive[0], []
+    for a, s in zip(arrive, sizes):
+        start = max(a, free_at)
+        free_at = start + 1e3 * pool.plain(s)
+        lat.append(free_at - a)
+    return len(sizes) / ((free_at - arrive[0]) / 1e3), np.array(lat)
+
+
 def serve_summary(pool, sizes, recs, makespan, stats, rate):
     """tomograms/s, p50/p99 latency, its breakdown per request class and the
-    plain per-series FP32 path on the same requests."""
+    plain per-series FP32 path on the same requests: at capacity (series back
+    to back) and on the same arrival times (FIFO, one series at a time)."""
     lat = np.array([r["latency_ms"] for r in recs])
     plain_total = sum(pool.plain(s) for s in sizes)
     tps = len(sizes) / makespan
     plain_tps = len(sizes) / plain_total
+    fifo_tps, fifo_lat = plain_fifo(pool, sizes, recs)
     updates = sum(pool.get(s)["updates"] for s in sizes)
 
     def part(sel):
@@ -348,6 +371,11 @@ def serve_summary(pool, sizes, recs, makespan, stats, rate):
                      "batching_sizes": part([r for r in recs if r["size"][1] != 2048])},
         "plain_fp32_per_series_tomograms_per_s": round(plain_tps, 3),
         "speedup_vs_plain": round(tps / plain_tps, 3),
+        "plain_fp32_same_arrivals": {
+            "tomograms_per_s": round(fifo_tps, 3),
+            "latency_ms_p50": round(float(np.percentile(fifo_lat, 50)), 1),
+            "latency_ms_p99": round(float(np.percentile(fifo_lat, 99)), 1)},
+        "speedup_vs_plain_same_arrivals": round(tps / fifo_tps, 3),
     }
 
 
@@ -357,7 +385,7 @@ SERVE_WORKLOAD = ("serve: open-loop Poisson stream, WBP + SART %d iterations x %
                   "shifts +-3 px, ~10%% large (2048^2 x 61 x 512), FP16 projection storage, FP32 "
                   "accumulation, admission by the device-memory budget (288 GB card), pinned host "
                   "buffers; plain = the same requests one after another through the per-series "
-                  "FP32 path (upload, WBP, SART, copy-back)")
+                  "FP32 path (plans, upload, WBP, SART, copy-back)")
 
 
 def serve_bench(args):
@@ -366,7 +394,8 @@ def serve_bench(args):
     every rate.  Prints one JSON line with a serving block per rate."""
     c = dict(SERVE)
     n_req = args.requests or c["requests"]
-    rates = [args.rate] if args.rate else [float(r) for r in c["rates"].split(",")]
+    rates = ([args.rate] if args.rate else
+             [float(r) for r in (args.rates or c["rates"]).split(",")])
     pool = ServePool(c["seed"], c["iters"], c["subsets"])
     sizes = serve_sizes(np.random.default_rng(c["seed"]), n_req, c["large_frac"])
     for sz in sorted(set(sizes)):
@@ -375,7 +404,8 @@ def serve_bench(args):
     per_rate = []
     for rate in rates:
         recs, makespan, stats, wall = serve_stream(pool, sizes, rate, c["seed"] + int(rate * 100),
-                                                   args.max_active or c["max_active"])
+                                                   args.max_active or c["max_active"],
+                                                   SCHEDULES[args.schedule])
         summ = serve_summary(pool, sizes, recs, makespan, stats, rate)
         per_rate.append(summ)
         print(f"[bench] serve rate {rate}/s: {json.dumps(summ)}", file=sys.stderr, flush=True)
@@ -397,7 +427,8 @@ def serve_bench(args):
         "config": {"workload": SERVE_WORKLOAD % (c["iters"], c["subsets"]) +
                    "; %d requests per rate, rates %s/s" % (n_req, ",".join("%g" % r for r in rates)),
                    "updates_per_stream": updates, "projection_storage": "f16",
-                   "max_active": args.max_active or c["max_active"]},
+                   "max_active": args.max_active or c["max_active"],
+                   "schedule": args.schedule},
         "serving": per_rate,
     }
     print(json.dumps(result))
@@ -724,7 +755,10 @@ def main():
     ap.add_argument("--config", default="sart1024", choices=sorted(CONFIGS) + ["serve", "batch16", "extras"])
     ap.add_argument("--requests", type=int, default=0, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second")
+    ap.add_argument("--rates", default="", help="serve: comma-separated arrival rates per second")
     ap.add_argument("--max-active", type=int, default=0, help="serve: series per round at most")
+    ap.add_argument("--schedule", default="fair", choices=sorted(SCHEDULES),
+                    help="serve: which series step in a round (fair share or every series)")
     ap.add_argument("--no-serve", action="store_true",
                     help="default run: leave out the serving block")
     ap.add_argument("--small", action="store_true", help="same as --config small")

@@ -237,28 +237,38 @@ void atr_batch_destroy(atr_batch* batch);
  * bytes fit the HBM budget, keeps each series' data on the GPU from
  * pre-stage to output, and advances all admitted series together: every
  * round is one persistent forward-projection and one persistent
- * back-projection launch carrying one SART step of each series.  Requests
+ * back-projection launch carrying one SART step of each series whose turn
+ * it is (every series, or by fair share: ATR_SCHEDULE_*).  Requests
  * admitted mid-run join at the next round.  Plans are built on a host thread
  * of the server ahead of admission; input uploads, WBP reconstructions and
  * copy-out run on their own streams beside the rounds; device memory comes
  * from an arena held until atr_server_destroy.  Each result equals what the
  * series gives alone. */
 typedef struct atr_server atr_server;
+/* Which admitted series take a SART step in a round: fair share (default)
+ * gives every series the same GPU work per round (deficit round robin on the
+ * voxel-tilt updates of a step: a series whose steps cost m times those of
+ * the cheapest one steps every m-th round); every-step advances each series
+ * by one step every round. */
+#define ATR_SCHEDULE_FAIR_SHARE 0
+#define ATR_SCHEDULE_EVERY_STEP 1
 typedef struct atr_server_config {
     size_t hbm_budget_bytes; /* 0 = 90% of the free device memory at creation */
-    int max_active;          /* series reconstructed together at most (default 4), 0 = no limit */
+    int max_active;          /* series reconstructed together at most (default 8), 0 = no limit */
     int storage;             /* projection storage of every request (default F16) */
+    int schedule;            /* ATR_SCHEDULE_* (default fair share) */
 } atr_server_config;
 /* Times on the host clock, ms since the server started.  Per request:
  * queue = admit - submit (waiting for budget or a slot), GPU = compute_done -
- * admit (input upload, WBP and every SART round), copy-out = done -
- * compute_done (the volumes to host memory). */
+ * admit (input upload, WBP and every SART round; the WBP volume is copied out
+ * behind the WBP kernels, beside the rounds), copy-out = done - compute_done
+ * (the SART volume to host memory). */
 typedef struct atr_request_info {
     double submit_ms, admit_ms, done_ms;
     double latency_ms;                   /* done - submit */
     int status;
     double prepared_ms;                  /* host tables of its plans built */
-    double compute_done_ms;              /* its last GPU kernel finished */
+    double compute_done_ms;              /* its last SART kernel finished */
 } atr_request_info;
 void atr_server_config_init(atr_server_config* cfg);
 atr_status atr_server_create(const atr_server_config* cfg, atr_server** server);

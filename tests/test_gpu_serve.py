@@ -48,10 +48,11 @@ def alone(raw, tilts, vz, kw, method, storage):
 LEAD = (200, 256, 48, 3.0, 96, 10, 5, 86.0, True)
 
 
-def test_serving_matches_alone(gpu_available):
+@pytest.mark.parametrize("schedule", [L.SCHEDULE_FAIR_SHARE, L.SCHEDULE_EVERY_STEP])
+def test_serving_matches_alone(gpu_available, schedule):
     reqs = [series(r) for r in [LEAD] + REQS]
-    print("case serving mix", flush=True)
-    srv = Server(storage=L.STORAGE_F16)
+    print(f"case serving mix schedule={schedule}", flush=True)
+    srv = Server(storage=L.STORAGE_F16, schedule=schedule)
     ids = [srv.submit(raw, tilts, vz, with_wbp=wbp, **kw) for raw, tilts, vz, kw, wbp in reqs]
     results = [srv.wait(i) for i in ids]
     stats = srv.stats()
