@@ -44,7 +44,7 @@ EXPORTS = [
     "atr_calc_vol_thick", "atr_thickness_edges",
 ]
 BATCH_GRAPH = 1
-SCHEDULE_FAIR_SHARE, SCHEDULE_EVERY_STEP = 0, 1
+SCHEDULE_FAIR_SHARE, SCHEDULE_EVERY_STEP, SCHEDULE_SHORTEST_FIRST = 0, 1, 2
 
 
 class SeriesDesc(ctypes.Structure):

@@ -666,6 +666,9 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
     const int rowb = pitch * (int)sizeof(float2);
     const float xmax1 = (float)(vol_x - 1);    // clamp of x + 1 for masked samples
     const gptr<const float> V = vol + (size_t)y * vol_z * vol_x;
+    // the slice as a buffer resource (raw, 32-bit records)
+    const __amdgpu_buffer_rsrc_t vrsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)V, (short)0, vol_z * vol_x * (int)sizeof(float), 0x00020000);
     float acc[FP_MAXP], tf1[FP_MAXP];
     int cnt[FP_MAXP];
 #pragma unroll
@@ -723,31 +726,53 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
         // Staging: a thread owns whole columns of interpolation pairs
         // (V[z][x], V[z][x+1]-V[z][x]); column 0 is the guard for x in [-1, 0).
         // Each half of the rows is loaded first (independent, coalesced
-        // loads), then the pairs are formed from registers.
-        if (!PF)
-        for (int col = tid; col < pitch; col += FP_THREADS) {
-            const int xi = col - 1;                      // -1 .. VolX-1
-            const int xa = max(xi, 0), xb = min(xa + 1, vol_x - 1);
-            const gptr<const float> base = V + (size_t)zb * vol_x;
+        // loads), then the pairs are formed from registers.  The loads are
+        // buffer loads over the slice: the lane's byte offset is fixed for
+        // the launch and the row offset is a scalar, so they cost no address
+        // VALU (and a load past the slice would return 0, not fault).
+        // Entry e stores (V - e dV, dV); the guard (e = 0, x = -1) holds the
+        // extrapolated V[0] - dV, which is V[0] - 1 dV: the same formula with
+        // e taken as 1.
+        if (!PF) {
+            const int rows_u = __builtin_amdgcn_readfirstlane(rows);
+            const int zoff = __builtin_amdgcn_readfirstlane(zb) * vol_x * (int)sizeof(float);
+            const int rowb_s = ROWB > 0 ? ROWB : rowb;
+            for (int col = tid; col < pitch; col += FP_THREADS) {
+                const int xi = col - 1;                      // -1 .. VolX-1
+                const int xa = max(xi, 0), xb = min(xa + 1, vol_x - 1);
+                const int oa = xa * (int)sizeof(float), ob = xb * (int)sizeof(float);
+                const float ecol = (float)max(col, 1);
+                char* wp = (char*)band + col * (int)sizeof(float2);
+                // a batch of FP_STAGE_ROWS rows starting at row h; FULL: all
+                // of them exist (no per-row test)
+                auto batch = [&](int h, auto full) {
+                    constexpr bool FULL = decltype(full)::value;
+                    float va[FP_STAGE_ROWS], vb[FP_STAGE_ROWS];
 #pragma unroll
-            for (int h = 0; h < FP_BZ_MAX + 1; h += FP_STAGE_ROWS) {
-                float va[FP_STAGE_ROWS], vb[FP_STAGE_ROWS];
-#pragma unroll
-                for (int r = 0; r < FP_STAGE_ROWS; r++) {
-                    if (h + r < rows) {
-                        va[r] = base[(size_t)(h + r) * vol_x + xa];
-                        vb[r] = base[(size_t)(h + r) * vol_x + xb];
+                    for (int r = 0; r < FP_STAGE_ROWS; r++) {
+                        if (FULL || h + r < rows_u) {
+                            const int so = zoff + (h + r) * vol_x * (int)sizeof(float);
+                            va[r] = __uint_as_float(
+                                __builtin_amdgcn_raw_buffer_load_b32(vrsrc, oa, so, 0));
+                            vb[r] = __uint_as_float(
+                                __builtin_amdgcn_raw_buffer_load_b32(vrsrc, ob, so, 0));
+                        }
                     }
-                }
 #pragma unroll
-                for (int r = 0; r < FP_STAGE_ROWS; r++) {
-                    if (h + r < rows) {
-                        const float dx = vb[r] - va[r];
-                        const float v0 = xi < 0 ? va[r] - dx : va[r];
-                        // SENT: (V, dV); else (V - e dV, dV) for entry e = col
-                        band[(h + r) * pitch + col] =
-                            make_float2(SENT ? v0 : fmaf(-(float)col, dx, v0), dx);
+                    for (int r = 0; r < FP_STAGE_ROWS; r++) {
+                        if (FULL || h + r < rows_u) {
+                            const float dx = vb[r] - va[r];
+                            // SENT: (V, dV), the guard (V[0] - dV, dV)
+                            const float v0 = SENT ? (xi < 0 ? va[r] - dx : va[r])
+                                                  : fmaf(-ecol, dx, va[r]);
+                            *(float2*)(wp + (h + r) * rowb_s) = make_float2(v0, dx);
+                        }
                     }
+                };
+#pragma unroll
+                for (int h = 0; h < FP_BZ_MAX + 1; h += FP_STAGE_ROWS) {
+                    if (h + FP_STAGE_ROWS <= rows_u) batch(h, std::true_type{});
+                    else if (h < rows_u) batch(h, std::false_type{});
                 }
             }
         }
@@ -762,8 +787,14 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
                 if (d.y > 0) {
                     const cptr<float4> G = geo + d.x;
                     if (!SENT && d.w != 0) {
+                        // the window's samples summed from zero, then added
+                        // to the ray's sum: a fresh loop-carried register
+                        // instead of one lane of the 16-register sum array
+                        float part = 0.0f;
+                        int unused = 0;
                         fp_run<false, false, ROWB>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1,
-                                             make_int2(0, 0), acc[i], cnt[i]);
+                                                   make_int2(0, 0), part, unused);
+                        acc[i] += part;
                     } else {
                         int xd = tile * FP_WAVE + lane;
                         // keep the interval load inside the band loop: hoisted,
@@ -771,8 +802,12 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
                         asm volatile("" : "+v"(xd));
                         int2 rr = make_int2(0, 0);
                         if (xd < proj) rr = ldgo(fp_range + (size_t)t * proj + xd);
-                        fp_run<true, SENT, ROWB>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1, rr, acc[i],
-                                           cnt[i]);
+                        float part = 0.0f;
+                        int cpart = 0;
+                        fp_run<true, SENT, ROWB>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1, rr, part,
+                                                 cpart);
+                        acc[i] += part;
+                        if (SENT) cnt[i] += cpart;
                     }
                 }
             }

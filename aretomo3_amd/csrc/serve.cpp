@@ -40,6 +40,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr int kRing = 4;   // rounds whose item arrays may be in flight at once
+constexpr int kStarveRounds = 16;   // shortest-first: longest wait of a series, in rounds
 
 // Device arena: chunks from hipMalloc, kept until the arena is destroyed;
 // blocks handed out first-fit in address order and coalesced with their
@@ -156,6 +157,7 @@ struct Request {
     int step = 0, last = 0;  // SART steps: 0 initial BP, 1..last iterations
     double cost = 0.0;       // voxel-tilt updates of one SART step (fair-share rounds)
     double credit = 0.0;     // fair-share credit, in the same unit
+    int skipped = 0;         // consecutive rounds without a step (shortest-first)
 };
 
 }  // namespace
@@ -331,6 +333,7 @@ static atr_status start_sart(atr_server* s, Request& r)
     r.last = p.iters * p.n_subsets;
     r.cost = (double)p.vol_x * p.vol_y * p.vol_z * ((double)p.n / std::max(1, p.n_subsets));
     r.credit = 0.0;
+    r.skipped = 0;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "serving start");
     return ATR_OK;
@@ -363,7 +366,7 @@ static atr_status grow_slots(atr_server* s, int need)
 // One round: one SART step of every admitted series.  A series past its last
 // step gets its fill + layout on the round stream and goes to the finisher.
 //
-// Which series take a step (fair share, the default): deficit round robin
+// Which series take a step.  Fair share: deficit round robin
 // on the voxel-tilt updates of a step.  Every round each series earns the
 // step cost of the cheapest one and takes a step when its credit covers its
 // own cost, so the cheapest series steps every round and one whose steps
@@ -371,6 +374,11 @@ static atr_status grow_slots(atr_server* s, int need)
 // work per round, and a small series admitted beside a large one finishes in
 // about its own share of the time instead of at the large one's step rate.
 // ATR_SCHEDULE_EVERY_STEP: every series takes one step every round.
+// ATR_SCHEDULE_SHORTEST_FIRST (the default): the series closest to done
+// (least remaining work, and those within twice it) step; the rest wait, at
+// most kStarveRounds rounds in a row.  Small requests then pass large ones
+// instead of sharing their rounds (lowest median latency), and a large one
+// still advances at least every 17th round.
 static atr_status run_round(atr_server* s)
 {
     auto& act = s->active;
@@ -380,7 +388,22 @@ static atr_status run_round(atr_server* s)
     s->slot = (s->slot + 1) % kRing;
     if (s->slot_used[k]) ATR_HIP(hipEventSynchronize(s->ev_slot[k]));
     std::vector<char> take(act.size(), 1);
-    if (s->cfg.schedule == ATR_SCHEDULE_FAIR_SHARE) {
+    if (s->cfg.schedule == ATR_SCHEDULE_SHORTEST_FIRST) {
+        // the series with the least remaining work, and every series within
+        // twice that; a series passed over for kStarveRounds rounds in a row
+        // is taken regardless, so every series keeps advancing
+        double least = -1.0;
+        for (auto& rp : act) {
+            const double rem = (double)(rp->last - rp->step + 1) * rp->cost;
+            if (least < 0.0 || rem < least) least = rem;
+        }
+        for (size_t i = 0; i < act.size(); i++) {
+            Request& r = *act[i];
+            const double rem = (double)(r.last - r.step + 1) * r.cost;
+            take[i] = rem <= 2.0 * least || r.skipped >= kStarveRounds;
+            r.skipped = take[i] ? 0 : r.skipped + 1;
+        }
+    } else if (s->cfg.schedule == ATR_SCHEDULE_FAIR_SHARE) {
         double quantum = 0.0;
         for (auto& rp : act) quantum = quantum > 0.0 ? std::min(quantum, rp->cost) : rp->cost;
         for (size_t i = 0; i < act.size(); i++) {
@@ -642,7 +665,7 @@ void atr_server_config_init(atr_server_config* c)
     if (!c) return;
     std::memset(c, 0, sizeof(*c));
     c->max_active = 8;   // several series overlap; later arrivals queue (lower p50 than all-at-once)
-    c->schedule = ATR_SCHEDULE_FAIR_SHARE;
+    c->schedule = ATR_SCHEDULE_SHORTEST_FIRST;
     c->storage = ATR_STORAGE_F16;
 }
 
@@ -670,7 +693,8 @@ atr_status atr_server_create(const atr_server_config* cfg, atr_server** out)
     if (!cfg || !out) return fail(ATR_ERR_INVALID_ARG, "atr_server_create: NULL argument");
     if (cfg->storage < ATR_STORAGE_F32 || cfg->storage > ATR_STORAGE_BF16)
         return fail(ATR_ERR_INVALID_ARG, "atr_server_create: unknown storage");
-    if (cfg->schedule != ATR_SCHEDULE_FAIR_SHARE && cfg->schedule != ATR_SCHEDULE_EVERY_STEP)
+    if (cfg->schedule != ATR_SCHEDULE_FAIR_SHARE && cfg->schedule != ATR_SCHEDULE_EVERY_STEP &&
+        cfg->schedule != ATR_SCHEDULE_SHORTEST_FIRST)
         return fail(ATR_ERR_INVALID_ARG, "atr_server_create: unknown schedule");
     *out = nullptr;
     auto* s = new atr_server;

@@ -291,7 +291,7 @@ class Server:
     together with the others on the GPU.  Output arrays are allocated here."""
 
     def __init__(self, hbm_budget=0, max_active=None, storage=L.STORAGE_F16,
-                 schedule=L.SCHEDULE_FAIR_SHARE):
+                 schedule=L.SCHEDULE_SHORTEST_FIRST):
         cfg = L.ServerConfig()
         lib().atr_server_config_init(ctypes.byref(cfg))
         cfg.hbm_budget_bytes, cfg.storage = int(hbm_budget), int(storage)

@@ -171,7 +171,7 @@ def describe(name, w, vol):
              alg, w["storage"].upper()))
 
 
-SCHEDULES = {"fair": 0, "every": 1}   # ATR_SCHEDULE_FAIR_SHARE / _EVERY_STEP
+SCHEDULES = {"fair": 0, "every": 1, "shortest": 2}   # ATR_SCHEDULE_*
 SERVE = dict(requests=500, rates="1,2,5,10,20", seed=20261206, large_frac=0.1, iters=10, subsets=5,
              max_active=8)
 # the short serving stream of the default run: a C5-shaped stream sized to
@@ -280,7 +280,7 @@ class ServePool:
         return times[-1]
 
 
-def serve_stream(pool, sizes, rate, seed, max_active, schedule=0):
+def serve_stream(pool, sizes, rate, seed, max_active, schedule=SCHEDULES["shortest"]):
     """One open-loop stream: request k arrives after Exp(1/rate) gaps and is
     submitted to a fresh server (FP16 storage).  Returns the per-request
     records and the makespan (first arrival to last completion)."""
@@ -757,8 +757,9 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second")
     ap.add_argument("--rates", default="", help="serve: comma-separated arrival rates per second")
     ap.add_argument("--max-active", type=int, default=0, help="serve: series per round at most")
-    ap.add_argument("--schedule", default="fair", choices=sorted(SCHEDULES),
-                    help="serve: which series step in a round (fair share or every series)")
+    ap.add_argument("--schedule", default="shortest", choices=sorted(SCHEDULES),
+                    help="serve: which series step in a round (shortest-first, fair share or "
+                    "every series)")
     ap.add_argument("--no-serve", action="store_true",
                     help="default run: leave out the serving block")
     ap.add_argument("--small", action="store_true", help="same as --config small")

@@ -245,18 +245,21 @@ void atr_batch_destroy(atr_batch* batch);
  * from an arena held until atr_server_destroy.  Each result equals what the
  * series gives alone. */
 typedef struct atr_server atr_server;
-/* Which admitted series take a SART step in a round: fair share (default)
- * gives every series the same GPU work per round (deficit round robin on the
- * voxel-tilt updates of a step: a series whose steps cost m times those of
- * the cheapest one steps every m-th round); every-step advances each series
- * by one step every round. */
+/* Which admitted series take a SART step in a round: shortest-first
+ * (default) steps the series with the least remaining work (and those within
+ * twice it), any other one at least every 17th round; fair share gives every
+ * series the same GPU work per round (deficit round robin on the voxel-tilt
+ * updates of a step: a series whose steps cost m times those of the cheapest
+ * one steps every m-th round); every-step advances each series by one step
+ * every round. */
 #define ATR_SCHEDULE_FAIR_SHARE 0
 #define ATR_SCHEDULE_EVERY_STEP 1
+#define ATR_SCHEDULE_SHORTEST_FIRST 2
 typedef struct atr_server_config {
     size_t hbm_budget_bytes; /* 0 = 90% of the free device memory at creation */
     int max_active;          /* series reconstructed together at most (default 8), 0 = no limit */
     int storage;             /* projection storage of every request (default F16) */
-    int schedule;            /* ATR_SCHEDULE_* (default fair share) */
+    int schedule;            /* ATR_SCHEDULE_* (default shortest-first) */
 } atr_server_config;
 /* Times on the host clock, ms since the server started.  Per request:
  * queue = admit - submit (waiting for budget or a slot), GPU = compute_done -
