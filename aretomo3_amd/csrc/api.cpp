@@ -272,7 +272,7 @@ static atr_status sart_tables_host(Plan& p, int tz, int chunk, int bz, int pf)
     Geometry& g = p.geo;
     p.bp_tz = tz;
     p.bp_chunk = chunk;
-    p.fp_pf = pf && fp_pf_supported(p.vol_x) ? 1 : 0;
+    p.fp_pf = fp_mode_supported(pf, p.vol_x) ? pf : 0;
     build_bp_tiles(g.cos_sin, g.bp_lohi, p.n, p.vol_x, p.vol_z, 1.0, p.proj / 2.0, tz, g.bp_tiles);
     if (!build_fp_tables(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, bz, g))
         return fail(ATR_ERR_INVALID_ARG, "SART needs every tilt in (-90, 90) degrees");
@@ -777,9 +777,11 @@ static atr_status plan_autotune(Plan& p, hipStream_t s, int* tz_out, int* chunk_
         for (int bz : bzs) fps.push_back({bz, 0});
         if (fp_pf_supported(p.vol_x)) {
             std::vector<int> pfz;
-            fp_band_candidates(p.vol_x, p.vol_z, pfz, 1);
-            for (int bz : pfz) fps.push_back({bz, 1});
+            fp_band_candidates(p.vol_x, p.vol_z, pfz, FP_PF_DMA);
+            for (int bz : pfz) fps.push_back({bz, FP_PF_DMA});
         }
+        if (fp_rp_supported(p.vol_x))
+            for (int bz : bzs) fps.push_back({bz, FP_PF_REG});
         float best_fp = 1e30f;
         bool fp_ok = false;
         for (auto& c : fps) {
@@ -937,10 +939,15 @@ atr_status atr_plan_set_tile_shape(atr_plan* plan, int bp_tz, int bp_chunk, int 
     if (p.method != ATR_METHOD_SART) return fail(ATR_ERR_INVALID_ARG, "tile shapes are SART settings");
     if ((bp_tz != 8 && bp_tz != 16) || (bp_chunk != 8 && bp_chunk != 16))
         return fail(ATR_ERR_INVALID_ARG, "bp_tz and bp_chunk must be 8 or 16");
-    const int pf = fp_prefetch ? 1 : 0;
-    if (pf && !fp_pf_supported(p.vol_x))
+    const int pf = fp_prefetch;
+    if (pf < 0 || pf > FP_PF_REG)
+        return fail(ATR_ERR_INVALID_ARG, "fp_prefetch must be 0, 1 or 2");
+    if (pf == FP_PF_DMA && !fp_pf_supported(p.vol_x))
         return fail(ATR_ERR_INVALID_ARG, "the band-prefetching forward projector needs VolX a "
                                          "multiple of 256 (VolX " + std::to_string(p.vol_x) + ")");
+    if (pf == FP_PF_REG && !fp_rp_supported(p.vol_x))
+        return fail(ATR_ERR_INVALID_ARG, "the register-prefetching forward projector needs VolX "
+                                         "1024 or 2048 (VolX " + std::to_string(p.vol_x) + ")");
     const int lim = fp_band_rows(p.vol_x, pf);
     if (fp_rows < 1 || fp_rows > lim)
         return fail(ATR_ERR_INVALID_ARG,

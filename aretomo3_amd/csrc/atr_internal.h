@@ -41,8 +41,19 @@ constexpr int FP_STAGE_ROWS = 9;              // rows a thread loads before form
 // current band; the raw rows sit after the pair rows.  Needs VolX % 256 == 0
 // (a wave's load instruction covers 256 floats of one row).
 constexpr int FP_LDS_PF = 160 * 1024;
-inline int fp_waves(int pf) { return pf ? FP_WAVES - 1 : FP_WAVES; }
+// Register-prefetching variant (fp_pf = 2): every thread holds the next
+// band's rows of its columns in registers while the current band is
+// sampled; VolX 1024 (one column per thread, up to 18 rows) and 2048 (two
+// columns, up to 9 rows).
+constexpr int FP_PF_DMA = 1, FP_PF_REG = 2;
+inline int fp_waves(int pf) { return pf == FP_PF_DMA ? FP_WAVES - 1 : FP_WAVES; }
 inline bool fp_pf_supported(int vol_x) { return vol_x >= 256 && vol_x % 256 == 0; }
+inline bool fp_rp_supported(int vol_x) { return vol_x == 1024 || vol_x == 2048; }
+inline bool fp_mode_supported(int pf, int vol_x)
+{
+    return pf == 0 || (pf == FP_PF_DMA && fp_pf_supported(vol_x)) ||
+           (pf == FP_PF_REG && fp_rp_supported(vol_x));
+}
 
 enum Kernel {
     K_BP = 0,
@@ -182,7 +193,7 @@ struct Plan {
     int* d_vflag = nullptr;           // [vol_y] volume slice holds sentinels
     double* d_vstats = nullptr;       // [vol_y][2] fill statistics of volume slices
     int bp_tz = BP_TZ, bp_chunk = BP_CHUNK;   // back-projection tile shape (autotuned)
-    int fp_pf = 0;                    // forward projector: band-prefetching variant (autotuned)
+    int fp_pf = 0;                    // forward projector: 0 direct staging, FP_PF_DMA, FP_PF_REG (autotuned)
     // batches holding this plan: their item arrays and captured graphs point
     // at its tables, so the tables must not be rebuilt while this is > 0
     int batch_refs = 0;

@@ -355,7 +355,7 @@ namespace atr {
 // interpolation pairs must fit the LDS budget of one workgroup.
 int fp_band_rows(int vol_x, int pf)
 {
-    if (!pf) {
+    if (pf != FP_PF_DMA) {
         int rows = FP_LDS_BUDGET / ((vol_x + 1) * (int)sizeof(float2));
         return std::min(FP_BZ_MAX, std::max(1, rows - 1));
     }

@@ -636,7 +636,14 @@ __device__ __forceinline__ void fp_raw_load(const ATR_GLOBAL_AS float* __restric
     }
 }
 
-template <typename S, bool SENT, int ROWB = 0, bool PF = false>
+// RP > 0: the register-prefetching variant (VolX 1024, 2048 without empty
+// voxels; RP = columns per thread, 1 or 2).  Each thread loads the
+// next band's rows of its columns into registers right after the current
+// band's pairs are in LDS, so the loads are in flight while the band is
+// sampled; the pairs are formed from the registers after the band.  Entry
+// VolX (x = VolX - 1 with dV = 0) is written by the owner of entry VolX - 1
+// from its second value, so no thread stages a column of its own beyond RP.
+template <typename S, bool SENT, int ROWB = 0, bool PF = false, int RP = 0>
 __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ band, int gx, int y)
 {
     const gptr<const float> vol = as_gptr(A->vol);
@@ -692,6 +699,59 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
     // workgroups on the chip are spread out in time instead of all hitting
     // HBM at once.  (A ray's samples are summed band by band in this order.)
     const int rot = (int)(((unsigned)y * 5u + (unsigned)gx) % (unsigned)nb);
+    // RP: rows of the band held in registers, per column c: pa = V[z][xa],
+    // pb = V[z][xb] of column tid + c FP_THREADS
+    constexpr int RPC = RP > 0 ? RP : 1;
+    constexpr int RP_ROWS = RP > 0 ? (FP_BZ_MAX + 1) / RP : 1;
+    float pa[RPC][RP_ROWS], pb[RPC][RP_ROWS];
+    auto rp_load = [&](int zb_n, int rows_n, auto full) {
+        constexpr bool FULL = decltype(full)::value;
+        const int zoff = zb_n * vol_x * (int)sizeof(float);
+#pragma unroll
+        for (int c = 0; c < RPC; c++) {
+            const int col = tid + c * FP_THREADS;
+            // columns 0 .. VolX - 1 (the threads past them load nothing)
+            const int xa = max(col - 1, 0), xb = min(xa + 1, vol_x - 1);
+            if (col >= vol_x) continue;
+#pragma unroll
+            for (int r = 0; r < RP_ROWS; r++) {
+                if (FULL || r < rows_n) {
+                    const int so = zoff + r * vol_x * (int)sizeof(float);
+                    pa[c][r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                        vrsrc, xa * (int)sizeof(float), so, 0));
+                    pb[c][r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                        vrsrc, xb * (int)sizeof(float), so, 0));
+                }
+            }
+        }
+    };
+    auto rp_load_band = [&](int b_n) {
+        const int zb_n = __builtin_amdgcn_readfirstlane(b_n * bz);
+        const int rows_n = __builtin_amdgcn_readfirstlane(min(bz, vol_z - 1 - zb_n) + 1);
+        if (rows_n == RP_ROWS) rp_load(zb_n, rows_n, std::true_type{});
+        else rp_load(zb_n, rows_n, std::false_type{});
+    };
+    auto rp_write = [&](int rows_n, auto full) {
+        constexpr bool FULL = decltype(full)::value;
+#pragma unroll
+        for (int c = 0; c < RPC; c++) {
+            const int col = tid + c * FP_THREADS;
+            if (col < vol_x) {
+                const float ecol = (float)max(col, 1);
+                char* wp = (char*)band + col * (int)sizeof(float2);
+                const bool last = col == vol_x - 1;
+#pragma unroll
+                for (int r = 0; r < RP_ROWS; r++) {
+                    if (FULL || r < rows_n) {
+                        const float dx = pb[c][r] - pa[c][r];
+                        *(float2*)(wp + r * ROWB) = make_float2(fmaf(-ecol, dx, pa[c][r]), dx);
+                        if (last) *(float2*)(wp + r * ROWB + 8) = make_float2(pb[c][r], 0.0f);
+                    }
+                }
+            }
+        }
+    };
+    if (RP) rp_load_band(rot);
     if (PF) {
         if (loader) {
             const int zb = rot * bz;
@@ -733,7 +793,13 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
         // Entry e stores (V - e dV, dV); the guard (e = 0, x = -1) holds the
         // extrapolated V[0] - dV, which is V[0] - 1 dV: the same formula with
         // e taken as 1.
-        if (!PF) {
+        if (RP) {
+            const int rows_u = __builtin_amdgcn_readfirstlane(rows);
+            if (rows_u == RP_ROWS) rp_write(rows_u, std::true_type{});
+            else rp_write(rows_u, std::false_type{});
+            __syncthreads();
+            if (bi + 1 < nb) rp_load_band(b + 1 < nb ? b + 1 : 0);
+        } else if (!PF) {
             const int rows_u = __builtin_amdgcn_readfirstlane(rows);
             const int zoff = __builtin_amdgcn_readfirstlane(zb) * vol_x * (int)sizeof(float);
             const int rowb_s = ROWB > 0 ? ROWB : rowb;
@@ -776,7 +842,7 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
                 }
             }
         }
-        if (!PF) __syncthreads();
+        if (!PF && !RP) __syncthreads();
 #pragma unroll
         for (int i = 0; i < FP_MAXP; i++) {
             const int jt = pair_tab[i];
@@ -838,11 +904,12 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
 // ROWB: the band row pitch in bytes, 8 (VolX + 1), when the launch fixes it
 // at compile time (VolX 512, 1024, 2048), else 0.
 // PF: the band-prefetching variant (VolX a multiple of 256).
-template <typename S, bool SENT, int ROWB, bool PF = false>
+// RP: the register-prefetching variant (columns per thread).
+template <typename S, bool SENT, int ROWB, bool PF = false, int RP = 0>
 __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(const FpArgs* __restrict__ a)
 {
     extern __shared__ float2 band[];
-    fp_block<S, SENT, ROWB, PF>(as_cptr(a), band, blockIdx.x, blockIdx.y);
+    fp_block<S, SENT, ROWB, PF, RP>(as_cptr(a), band, blockIdx.x, blockIdx.y);
 }
 
 // Forward projection + difference of many series in one persistent launch:
@@ -1139,7 +1206,7 @@ static size_t fp_pair_bytes(int bz, int vol_x) { return (size_t)(bz + 1) * (vol_
 size_t fp_lds_bytes(int bz, int vol_x, int pf)
 {
     const size_t pairs = fp_pair_bytes(bz, vol_x);
-    if (!pf) return pairs;
+    if (pf != FP_PF_DMA) return pairs;
     return (pairs + 255) / 256 * 256 + (size_t)(bz + 1) * vol_x * sizeof(float);
 }
 
@@ -1158,12 +1225,12 @@ static void allow_big_lds(K kernel, std::atomic<unsigned long long>& done, int b
         done.fetch_or(bit, std::memory_order_relaxed);
 }
 
-template <typename S, bool SENT, int ROWB, bool PF = false>
+template <typename S, bool SENT, int ROWB, bool PF = false, int RP = 0>
 static void launch_fp_k(dim3 grid, size_t shm, const FpArgs* a, hipStream_t s)
 {
     static std::atomic<unsigned long long> done{0};
-    allow_big_lds(k_for_proj_diff<S, SENT, ROWB, PF>, done, PF ? FP_LDS_PF : FP_LDS_BUDGET);
-    hipLaunchKernelGGL((k_for_proj_diff<S, SENT, ROWB, PF>), grid, dim3(FP_THREADS), shm, s, a);
+    allow_big_lds(k_for_proj_diff<S, SENT, ROWB, PF, RP>, done, PF ? FP_LDS_PF : FP_LDS_BUDGET);
+    hipLaunchKernelGGL((k_for_proj_diff<S, SENT, ROWB, PF, RP>), grid, dim3(FP_THREADS), shm, s, a);
 }
 
 template <typename S>
@@ -1176,7 +1243,13 @@ static void launch_fp_t(const Plan& p, int subset, hipStream_t s)
     const FpArgs* a = p.d_fp_args + subset;
     // the sentinel-free slices (nearly all) take a variant with the band
     // pitch fixed at compile time for the common widths
-    if (p.fp_pf) {
+    if (p.fp_pf == FP_PF_REG) {   // VolX 1024 or 2048
+        if (p.vol_x == 1024) launch_fp_k<S, false, 1025 * 8, false, 1>(grid, shm, a, s);
+        else launch_fp_k<S, false, 2049 * 8, false, 2>(grid, shm, a, s);
+        launch_fp_k<S, true, 0>(grid, shm, a, s);
+        return;
+    }
+    if (p.fp_pf == FP_PF_DMA) {
         switch (p.vol_x) {
         case 512: launch_fp_k<S, false, 513 * 8, true>(grid, shm, a, s); break;
         case 1024: launch_fp_k<S, false, 1025 * 8, true>(grid, shm, a, s); break;

@@ -166,7 +166,7 @@ class Plan:
 
     def set_tile_shape(self, bp_tz, bp_chunk, fp_band_rows, fp_prefetch=0):
         check(lib().atr_plan_set_tile_shape(self.handle, bp_tz, bp_chunk, fp_band_rows,
-                                            1 if fp_prefetch else 0))
+                                            int(fp_prefetch)))
 
     @property
     def volume_shape(self):

@@ -159,10 +159,12 @@ atr_status atr_execute_host(atr_plan* plan, const float* proj_host, float* vol_h
  * launch constants, Recon/GBackProj.cu:79-82, Recon/GForProj.cu:106-110):
  * times the candidate tile shapes of a SART plan on its own geometry -- back-
  * projection z rows per thread {16, 8} x tilts per LDS stage {16, 8}, then
- * forward-projection band rows, for the direct-staging forward projector and
+ * forward-projection band rows, for the direct-staging forward projector,
  * (VolX a multiple of 256) the band-prefetching one (fp_prefetch = 1: one
- * wave streams the next z-band into LDS while the others sample) -- and keeps
- * the fastest.  Choices are cached per shape within the process.  Every voxel
+ * wave streams the next z-band into LDS while the others sample) and (VolX
+ * 1024, 2048) the register-prefetching one (fp_prefetch = 2: every
+ * thread loads its columns of the next z-band into registers while the band
+ * is sampled) -- and keeps the fastest.  Choices are cached per shape within the process.  Every voxel
  * sums its tilts in the same order whatever the shape; the FP32 rounding of
  * the back-projection coordinate (taken relative to the tile's window) and
  * the order in which a forward ray adds up its z-bands depend on the shape,
@@ -172,9 +174,9 @@ atr_status atr_execute_host(atr_plan* plan, const float* proj_host, float* vol_h
  * until atr_batch_destroy) cannot be retuned: ATR_ERR_INVALID_ARG. */
 atr_status atr_plan_autotune(atr_plan* plan, void* stream, int* bp_tz, int* bp_chunk,
                              int* fp_band_rows, int* fp_prefetch);
-/* Sets the tile shapes by hand: bp_tz, bp_chunk in {8, 16}; fp_prefetch 0 or
- * 1 (1 needs VolX a multiple of 256); fp_band_rows in [1, the largest band
- * that fits in LDS for that forward projector].  Refused for a plan that
+/* Sets the tile shapes by hand: bp_tz, bp_chunk in {8, 16}; fp_prefetch 0, 1
+ * (VolX a multiple of 256) or 2 (VolX 1024 or 2048); fp_band_rows in
+ * [1, the largest band that fits in LDS for that forward projector].  Refused for a plan that
  * belongs to a batch, as atr_plan_autotune. */
 atr_status atr_plan_set_tile_shape(atr_plan* plan, int bp_tz, int bp_chunk, int fp_band_rows,
                                    int fp_prefetch);

@@ -69,44 +69,50 @@ def test_autotune(gpu_available):
 
 
 @pytest.mark.parametrize("px,storage", [(512, L.STORAGE_F32), (768, L.STORAGE_F16),
-                                        (1024, L.STORAGE_F32)])
+                                        (1024, L.STORAGE_F32), (2048, L.STORAGE_F16)])
 def test_prefetch_forward_projector(gpu_available, px, storage):
     # the band-prefetching forward projector (one wave streams the next band
-    # into LDS) gives the direct-staging projector's volume bit for bit at the
-    # same band rows, for widths 512 / 768 / 1024 (compile-time and run-time
-    # band pitch), with and without empty voxels in the volume
+    # into LDS) and the register-prefetching one (VolX 1024 / 2048: every
+    # thread loads its columns of the next band into registers) give the
+    # direct-staging projector's volume bit for bit at the same band rows, for
+    # widths 512 / 768 / 1024 / 2048 (compile-time and run-time band pitch),
+    # with and without empty voxels in the volume
+    modes = [0] + [1] * (px % 256 == 0) + [2] * (px in (1024, 2048))
+    rows = 5 if px == 2048 else 8     # band rows every mode fits (LDS)
     tilts = phantom.tilt_range(-60, 60, 4)
     proj, _ = phantom.tilt_series(91, px, 3, tilts, 96, n_ellipsoids=60, axis_min=6, axis_max=40,
                                   noise_frac=0.05)
-    print(f"case prefetch fp px={px} storage={storage}", flush=True)
+    print(f"case prefetch fp px={px} storage={storage} modes={modes}", flush=True)
     kw = dict(sart_iters=2, sart_subsets=3, relax=0.3, subset_order=L.SUBSETS_INTERLEAVED,
               positivity=False, storage=storage)
     plan = Plan(px, 3, tilts, 96, L.METHOD_SART, **kw)
     out = {}
-    for pf in (0, 1):
-        plan.set_tile_shape(16, 16, 9, pf)
+    for pf in modes:
+        plan.set_tile_shape(16, 16, rows, pf)
         out[pf] = plan.execute_host(proj)
     plan.close()
-    assert np.array_equal(out[0], out[1])
+    for pf in modes[1:]:
+        assert np.array_equal(out[0], out[pf]), pf
     ref = oracle.recon_slices(proj.astype(np.float64), tilts, 1, 96, [0, 1, 2], iters=2,
                               n_subsets=3, relax=0.3, order=1, positivity=0)
-    err = rel_l2(out[1], ref)
+    err = rel_l2(out[modes[-1]], ref)
     print("rel_l2", err)
     assert err <= (TOL32 if storage == L.STORAGE_F32 else 2e-3)
     # with empty voxels: the slice operator continuing from a volume that holds
-    # -1e30 voxels runs the sentinel variant of both projectors
+    # -1e30 voxels runs the sentinel variant beside each projector
     start = np.random.default_rng(5).random((3, 96, px)).astype(np.float32)
     start[1, 40:44, 100:140] = -1e30
     res = {}
-    for pf in (0, 1):
+    for pf in modes:
         p = Plan(px, 3, tilts, 96, L.METHOD_SART, layout=L.LAYOUT_SLICE, **kw)
-        p.set_tile_shape(16, 16, 9, pf)
+        p.set_tile_shape(16, 16, rows, pf)
         d_in = DeviceArray.from_numpy(proj)
         d_vol = DeviceArray.from_numpy(start)
         p.execute(d_in, d_vol, accumulate=True)
         res[pf] = d_vol.to_numpy(start.shape)
         p.close()
-    assert np.array_equal(res[0], res[1])
+    for pf in modes[1:]:
+        assert np.array_equal(res[0], res[pf]), pf
 
 
 def test_prefetch_refused_for_other_widths(gpu_available):
@@ -115,4 +121,8 @@ def test_prefetch_refused_for_other_widths(gpu_available):
     print("case prefetch refusal", flush=True)
     with pytest.raises(L.AtrError, match="multiple of 256"):
         plan.set_tile_shape(16, 16, 4, 1)
+    with pytest.raises(L.AtrError, match="1024 or 2048"):
+        plan.set_tile_shape(16, 16, 4, 2)
+    with pytest.raises(L.AtrError, match="0, 1 or 2"):
+        plan.set_tile_shape(16, 16, 4, 3)
     plan.close()
