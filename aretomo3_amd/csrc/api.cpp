@@ -385,8 +385,12 @@ static atr_status plan_host(Plan& p)
         h.lists = p.all_tilts;
         h.lists.insert(h.lists.end(), p.subset_tilts.begin(), p.subset_tilts.end());
         fill_table(d.fill_seed ^ 0x5A5A5A5A5A5A5A5Aull, p.vol_x * p.vol_z, h.fill_vol);
+        // default forward-projector staging: register prefetch where it is
+        // built (VolX 1024, 2048: measured faster at the full band height),
+        // else direct
         return sart_tables_host(p, p.bp_tz, p.bp_chunk,
-                                fp_balanced_rows(fp_band_rows(p.vol_x), p.vol_z), 0);
+                                fp_balanced_rows(fp_band_rows(p.vol_x), p.vol_z),
+                                fp_rp_supported(p.vol_x) ? FP_PF_REG : 0);
     }
     build_wbp_intervals(g.cos_sin, p.n, p.vol_x, p.vol_z, p.proj, g.wbp_lohi);
     // 2x-grid back-projection: tile windows/classes (excluded tilts are left

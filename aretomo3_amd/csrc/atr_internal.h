@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <functional>
@@ -264,6 +265,32 @@ struct BpItem {
 };
 // 0: 16 z rows x 16 tilts per stage, 1: 16 x 8, 2: 8 x 16, 3: 8 x 8
 inline int bp_shape(int tz, int chunk) { return (tz == 8 ? 2 : 0) + (chunk == 8 ? 1 : 0); }
+
+// The entries of one persistent work-queue launch, laid out largest work
+// item first: the items the launch claims last are then the cheapest ones,
+// so it ends with a short tail instead of the big slices of whichever series
+// came last.  Items are independent, so the order changes no result.  Sets
+// item0, copies the entries to out, returns the number of queue items.
+template <class Item>
+struct QueueEntry {
+    Item item;
+    int n_items;       // queue items of the entry
+    double item_cost;  // work of one of them (any unit common to the launch)
+};
+template <class Item>
+int lay_out_entries(std::vector<QueueEntry<Item>>& v, Item* out)
+{
+    std::stable_sort(v.begin(), v.end(), [](const QueueEntry<Item>& a, const QueueEntry<Item>& b) {
+        return a.item_cost > b.item_cost;
+    });
+    int total = 0;
+    for (size_t i = 0; i < v.size(); i++) {
+        v[i].item.item0 = total;
+        total += v[i].n_items;
+        out[i] = v[i].item;
+    }
+    return total;
+}
 struct FpItem {
     FpArgs a;
     int item0, n_groups;       // items of one slice = tilt groups

@@ -68,25 +68,28 @@ static atr_status batch_build(atr_batch* b)
     // initial back-projection of every series (relax 1, CTomoSart.cpp:68-69)
     {
         atr_batch::Step st{false, (int)bp.size(), 0, 0, b->n_counters++};
+        std::vector<QueueEntry<BpItem>> ent;
         for (atr_plan* h : b->plans) {
             const Plan& p = h->p;
             const int n_all = (int)p.all_tilts.size();
             if (n_all == 0) continue;
             BpItem e{};
             e.a = bp_args(p, p.d_sino, p.n, p.d_lists, n_all, 1.0f, true);
-            e.item0 = st.total;
             e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
             e.shape = bp_shape(p.bp_tz, p.bp_chunk);
-            st.total += p.vol_y * e.n_items_y;
-            bp.push_back(e);
+            ent.push_back({e, p.vol_y * e.n_items_y, (double)n_all * p.bp_tz});
             st.n_entries++;
         }
+        bp.resize(st.first + ent.size());
+        st.total = lay_out_entries(ent, bp.data() + st.first);
         if (st.n_entries) b->steps.push_back(st);
     }
     for (int it = 0; it < max_iters; it++)
         for (int sb = 0; sb < max_sub; sb++) {
             atr_batch::Step sf{true, (int)fp.size(), 0, 0, b->n_counters++};
             atr_batch::Step sp{false, (int)bp.size(), 0, 0, b->n_counters++};
+            std::vector<QueueEntry<FpItem>> fent;
+            std::vector<QueueEntry<BpItem>> bent;
             for (atr_plan* h : b->plans) {
                 const Plan& p = h->p;
                 if (it >= p.iters || sb >= p.n_subsets) continue;
@@ -95,23 +98,24 @@ static atr_status batch_build(atr_batch* b)
                 const int n_all = (int)p.all_tilts.size();
                 FpItem f{};
                 f.a = fp_args(p, p.d_fp_lists + f0, f1 - f0, p.d_fp_pairs + p.subset_pair_off[sb]);
-                f.item0 = sf.total;
                 f.n_groups = (f1 - f0 + f.a.group - 1) / f.a.group;
-                sf.total += p.vol_y * f.n_groups;
-                fp.push_back(f);
+                fent.push_back({f, p.vol_y * f.n_groups,
+                                (double)p.vol_x * p.vol_z * std::min(f.a.group, f1 - f0)});
                 sf.n_entries++;
                 if (b1 > b0) {
                     BpItem e{};
                     e.a = bp_args(p, p.d_diff, p.max_subset, p.d_lists + n_all + b0, b1 - b0,
                                   p.relax, false);
-                    e.item0 = sp.total;
                     e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
                     e.shape = bp_shape(p.bp_tz, p.bp_chunk);
-                    sp.total += p.vol_y * e.n_items_y;
-                    bp.push_back(e);
+                    bent.push_back({e, p.vol_y * e.n_items_y, (double)(b1 - b0) * p.bp_tz});
                     sp.n_entries++;
                 }
             }
+            fp.resize(sf.first + fent.size());
+            sf.total = lay_out_entries(fent, fp.data() + sf.first);
+            bp.resize(sp.first + bent.size());
+            sp.total = lay_out_entries(bent, bp.data() + sp.first);
             if (sf.n_entries) b->steps.push_back(sf);
             if (sp.n_entries) b->steps.push_back(sp);
         }

@@ -935,10 +935,21 @@ __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff_batch(const FpItem
         if (as_cptr(A->vflag)[y] != 0) {
             fp_block<S, true>(A, band, gx, y);
         } else {
+            // the series' own staging: register prefetch where its plan
+            // chose it (VolX 1024 / 2048), else direct (the band-prefetching
+            // variant's loader wave does not fit a shared launch; every mode
+            // gives the same pairs)
+            const bool rp = A->pf == FP_PF_REG;
             switch (A->vol_x) {
             case 512: fp_block<S, false, 513 * 8>(A, band, gx, y); break;
-            case 1024: fp_block<S, false, 1025 * 8>(A, band, gx, y); break;
-            case 2048: fp_block<S, false, 2049 * 8>(A, band, gx, y); break;
+            case 1024:
+                if (rp) fp_block<S, false, 1025 * 8, false, 1>(A, band, gx, y);
+                else fp_block<S, false, 1025 * 8>(A, band, gx, y);
+                break;
+            case 2048:
+                if (rp) fp_block<S, false, 2049 * 8, false, 2>(A, band, gx, y);
+                else fp_block<S, false, 2049 * 8>(A, band, gx, y);
+                break;
             default: fp_block<S, false, 0>(A, band, gx, y); break;
             }
         }
@@ -1043,14 +1054,23 @@ __global__ __launch_bounds__(256) void k_output(const float* __restrict__ vol, i
     out[out_index(layout, x, y, z, vol_x, vol_y, vol_z)] = v;
 }
 
-// Plain streaming copy (16 B per lane, grid-stride): the calibration kernel
-// for HBM byte counters.
+// Plain streaming copy (16 B per lane, grid-stride, four independent loads
+// in flight per lane): the calibration kernel for HBM byte counters and the
+// attainable-bandwidth reference.
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src,
                                               float4* __restrict__ dst, size_t n)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride],
+                     d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
 }
 
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s)

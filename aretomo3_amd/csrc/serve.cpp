@@ -413,8 +413,10 @@ static atr_status run_round(atr_server* s)
             if (take[i]) r.credit -= r.cost;
         }
     }
-    int n_fp = 0, n_bp = 0, tot_fp = 0, tot_bp = 0, n_take = 0;
+    int n_take = 0;
     size_t shm = 0;
+    std::vector<QueueEntry<FpItem>> fent;
+    std::vector<QueueEntry<BpItem>> bent;
     for (size_t i = 0; i < act.size(); i++) {
         if (!take[i]) continue;
         n_take++;
@@ -424,34 +426,32 @@ static atr_status run_round(atr_server* s)
         if (r.step == 0) {
             BpItem e{};
             e.a = bp_args(p, p.d_sino, p.n, p.d_lists, n_all, 1.0f, true);
-            e.item0 = tot_bp;
             e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
             e.shape = bp_shape(p.bp_tz, p.bp_chunk);
-            tot_bp += p.vol_y * e.n_items_y;
-            s->h_bp[k][n_bp++] = e;
+            bent.push_back({e, p.vol_y * e.n_items_y, (double)n_all * p.bp_tz});
         } else if (r.step <= r.last) {
             const int sb = (r.step - 1) % p.n_subsets;
             const int f0 = p.subset_fp_off[sb], f1 = p.subset_fp_off[sb + 1];
             const int b0 = p.subset_off[sb], b1 = p.subset_off[sb + 1];
             FpItem f{};
             f.a = fp_args(p, p.d_fp_lists + f0, f1 - f0, p.d_fp_pairs + p.subset_pair_off[sb]);
-            f.item0 = tot_fp;
             f.n_groups = (f1 - f0 + f.a.group - 1) / f.a.group;
-            tot_fp += p.vol_y * f.n_groups;
-            s->h_fp[k][n_fp++] = f;
+            fent.push_back({f, p.vol_y * f.n_groups,
+                            (double)p.vol_x * p.vol_z * std::min(f.a.group, f1 - f0)});
             shm = std::max(shm, fp_lds_bytes(p.geo.fp_bz, p.vol_x));
             if (b1 > b0) {
                 BpItem e{};
                 e.a = bp_args(p, p.d_diff, p.max_subset, p.d_lists + n_all + b0, b1 - b0, p.relax,
                               false);
-                e.item0 = tot_bp;
                 e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
                 e.shape = bp_shape(p.bp_tz, p.bp_chunk);
-                tot_bp += p.vol_y * e.n_items_y;
-                s->h_bp[k][n_bp++] = e;
+                bent.push_back({e, p.vol_y * e.n_items_y, (double)(b1 - b0) * p.bp_tz});
             }
         }
     }
+    const int n_fp = (int)fent.size(), n_bp = (int)bent.size();
+    const int tot_fp = lay_out_entries(fent, s->h_fp[k]);
+    const int tot_bp = lay_out_entries(bent, s->h_bp[k]);
     hipStream_t q = s->s_comp;
     launch_zero_int(s->d_cnt[k], 3, q);
     if (n_fp) {
