@@ -41,7 +41,9 @@ constexpr int FP_STAGE_ROWS = 9;              // rows a thread loads before form
 // into LDS with direct global->LDS loads while the other waves sample the
 // current band; the raw rows sit after the pair rows.  Needs VolX % 256 == 0
 // (a wave's load instruction covers 256 floats of one row).
-constexpr int FP_LDS_PF = 160 * 1024;
+// (the 160 KiB of a CU less 256 bytes for the static LDS of the batched
+// work-queue kernel, which runs these bands too)
+constexpr int FP_LDS_PF = 160 * 1024 - 256;
 // Register-prefetching variant (fp_pf = 2): every thread holds the next
 // band's rows of its columns in registers while the current band is
 // sampled; VolX 1024 (one column per thread, up to 18 rows) and 2048 (two

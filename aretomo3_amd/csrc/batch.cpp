@@ -63,7 +63,7 @@ static atr_status batch_build(atr_batch* b)
     for (atr_plan* h : b->plans) {
         max_iters = std::max(max_iters, h->p.iters);
         max_sub = std::max(max_sub, h->p.n_subsets);
-        b->fp_shm = std::max(b->fp_shm, fp_lds_bytes(h->p.geo.fp_bz, h->p.vol_x));
+        b->fp_shm = std::max(b->fp_shm, fp_lds_bytes(h->p.geo.fp_bz, h->p.vol_x, h->p.fp_pf));
     }
     // initial back-projection of every series (relax 1, CTomoSart.cpp:68-69)
     {

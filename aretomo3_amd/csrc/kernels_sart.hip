@@ -935,22 +935,30 @@ __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff_batch(const FpItem
         if (as_cptr(A->vflag)[y] != 0) {
             fp_block<S, true>(A, band, gx, y);
         } else {
-            // the series' own staging: register prefetch where its plan
-            // chose it (VolX 1024 / 2048), else direct (the band-prefetching
-            // variant's loader wave does not fit a shared launch; every mode
-            // gives the same pairs)
-            const bool rp = A->pf == FP_PF_REG;
-            switch (A->vol_x) {
-            case 512: fp_block<S, false, 513 * 8>(A, band, gx, y); break;
-            case 1024:
-                if (rp) fp_block<S, false, 1025 * 8, false, 1>(A, band, gx, y);
-                else fp_block<S, false, 1025 * 8>(A, band, gx, y);
-                break;
-            case 2048:
-                if (rp) fp_block<S, false, 2049 * 8, false, 2>(A, band, gx, y);
-                else fp_block<S, false, 2049 * 8>(A, band, gx, y);
-                break;
-            default: fp_block<S, false, 0>(A, band, gx, y); break;
+            // the staging mode of the series' own plan (its band rows and
+            // wave schedule were built for it), so a series computes here
+            // exactly what it computes alone
+            if (A->pf == FP_PF_DMA) {
+                switch (A->vol_x) {
+                case 512: fp_block<S, false, 513 * 8, true>(A, band, gx, y); break;
+                case 1024: fp_block<S, false, 1025 * 8, true>(A, band, gx, y); break;
+                case 2048: fp_block<S, false, 2049 * 8, true>(A, band, gx, y); break;
+                default: fp_block<S, false, 0, true>(A, band, gx, y); break;
+                }
+            } else {
+                const bool rp = A->pf == FP_PF_REG;
+                switch (A->vol_x) {
+                case 512: fp_block<S, false, 513 * 8>(A, band, gx, y); break;
+                case 1024:
+                    if (rp) fp_block<S, false, 1025 * 8, false, 1>(A, band, gx, y);
+                    else fp_block<S, false, 1025 * 8>(A, band, gx, y);
+                    break;
+                case 2048:
+                    if (rp) fp_block<S, false, 2049 * 8, false, 2>(A, band, gx, y);
+                    else fp_block<S, false, 2049 * 8>(A, band, gx, y);
+                    break;
+                default: fp_block<S, false, 0>(A, band, gx, y); break;
+                }
             }
         }
         it = publish_item(ahead);
@@ -1240,9 +1248,11 @@ static void allow_big_lds(K kernel, std::atomic<unsigned long long>& done, int b
     const bool known = hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64;
     const unsigned long long bit = known ? 1ull << dev : 0ull;
     if (known && (done.load(std::memory_order_relaxed) & bit)) return;
-    if (hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes) == hipSuccess && known)
-        done.fetch_or(bit, std::memory_order_relaxed);
+    const hipError_t e = hipFuncSetAttribute((const void*)kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess && known) done.fetch_or(bit, std::memory_order_relaxed);
+    // a refusal surfaces as the launch's own error, not as a stale one
+    if (e != hipSuccess) (void)hipGetLastError();
 }
 
 template <typename S, bool SENT, int ROWB, bool PF = false, int RP = 0>
@@ -1299,7 +1309,7 @@ template <typename S>
 static void fp_batch_attrs()
 {
     static std::atomic<unsigned long long> done{0};
-    allow_big_lds(k_for_proj_diff_batch<S>, done);
+    allow_big_lds(k_for_proj_diff_batch<S>, done, FP_LDS_PF);   // the largest mode's LDS
 }
 
 template <typename S>

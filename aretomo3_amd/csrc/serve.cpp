@@ -438,7 +438,7 @@ static atr_status run_round(atr_server* s)
             f.n_groups = (f1 - f0 + f.a.group - 1) / f.a.group;
             fent.push_back({f, p.vol_y * f.n_groups,
                             (double)p.vol_x * p.vol_z * std::min(f.a.group, f1 - f0)});
-            shm = std::max(shm, fp_lds_bytes(p.geo.fp_bz, p.vol_x));
+            shm = std::max(shm, fp_lds_bytes(p.geo.fp_bz, p.vol_x, p.fp_pf));
             if (b1 > b0) {
                 BpItem e{};
                 e.a = bp_args(p, p.d_diff, p.max_subset, p.d_lists + n_all + b0, b1 - b0, p.relax,
