@@ -80,8 +80,8 @@ CONFIGS = {
 
 def measured_traffic():
     """HBM bytes per k_back_proj launch of the headline SART series from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by the
-    copy-kernel calibration), or None."""
+    committed rocprofv3 summary (128 B per TCC_EA0_RDREQ read request plus
+    WRITE_SIZE), or None."""
     try:
         text = open(os.path.join(ROOT, TRAFFIC_FILE)).read()
     except OSError:
@@ -924,8 +924,9 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "traffic_source": (TRAFFIC_FILE + " (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per "
-                               "launch, copy-kernel calibrated)") if traffic else None,
+            "traffic_source": (TRAFFIC_FILE + " (rocprofv3 per launch: 128 B x TCC_EA0_RDREQ_sum "
+                               "read + WRITE_SIZE written; FETCH_SIZE counts these 128-byte "
+                               "requests as 64 B)") if traffic else None,
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": round(avg_ms, 4),
             "measured_copy_peak": copy_peak,
