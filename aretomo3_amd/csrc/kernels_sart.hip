@@ -240,13 +240,19 @@ __device__ __forceinline__ void bp_tile_body(cptr<BpArgs> A, float2 (&pairs)[CH]
     const int x = x0 + tid;
     const int xc = min(x, vol_x - 1);
     const gptr<float> vslice = vol + (size_t)y * vol_z * vol_x;
+    // the slice (and below, the slice's sinogram rows) as buffers: a voxel's
+    // row offset is a scalar, its column the lane's fixed byte offset
+    const __amdgpu_buffer_rsrc_t vres =
+        buffer_of((const void*)vslice, (size_t)vol_z * vol_x * sizeof(float));
+    const int row_bytes = vol_x * (int)sizeof(float);
 
     // the voxels this thread updates, loaded now and consumed at the end
     // (the WBP grid is written, not updated)
     float vold[TZ];
 #pragma unroll
     for (int k = 0; k < TZ; k++)
-        vold[k] = WBP ? 0.0f : vslice[(size_t)min(z0 + k, vol_z - 1) * vol_x + xc];
+        vold[k] = WBP ? 0.0f
+                      : ldb<float>(vres, xc * (int)sizeof(float), min(z0 + k, vol_z - 1) * row_bytes);
 
     float acc[TZ];
     int cnt[TZ];
@@ -258,6 +264,8 @@ __device__ __forceinline__ void bp_tile_body(cptr<BpArgs> A, float2 (&pairs)[CH]
     int full_cnt = 0;
     const float xf = (float)(xc - x0);   // offset from the tile's first voxel
     const gptr<const S> base = sino + (size_t)y * rows_per_slice * stride;
+    const __amdgpu_buffer_rsrc_t sres =
+        buffer_of((const void*)base, (size_t)rows_per_slice * stride * sizeof(S));
     const cptr<int4> tile_tab = tiles + (size_t)zt * n_xtiles + xt;
     const size_t tilt_step = (size_t)n_ztiles * n_xtiles;
 
@@ -278,14 +286,14 @@ __device__ __forceinline__ void bp_tile_body(cptr<BpArgs> A, float2 (&pairs)[CH]
             for (int q = 0; q < 4; q++) {
                 const TiltRow tr = ldc(list + c0 + min(h + q, nc - 1));
                 const int w0 = tile_tab[(size_t)tr.tilt * tilt_step].x;
-                const gptr<const S> row = base + (size_t)tr.row * stride;
+                const int row_off = tr.row * stride * (int)sizeof(S);
 #pragma unroll
                 for (int m = 0; m < 2; m++) {
                     const int g = w0 + tid + m * BP_TX;
                     const int ga = min(max(g, 0), proj - 2);
                     if (m == 0 || tid < WIN - BP_TX) {
-                        pv[q][m] = ldg(row + (unsigned)ga);
-                        rv[q][m] = ldg(row + (unsigned)ga + 1u);
+                        pv[q][m] = ldb<S>(sres, ga * (int)sizeof(S), row_off);
+                        rv[q][m] = ldb<S>(sres, (ga + 1) * (int)sizeof(S), row_off);
                     }
                     gv[q][m] = g - ga;
                 }
@@ -370,7 +378,7 @@ __device__ __forceinline__ void bp_tile_body(cptr<BpArgs> A, float2 (&pairs)[CH]
             const int z = z0 + k;
             const int count = full_cnt + cnt[k];
             const float v = count > 0 ? acc[k] * __builtin_amdgcn_rcpf((float)count) : 0.0f;
-            if (x < vol_x && z < vol_z) vslice[(size_t)z * vol_x + x] = v;
+            if (x < vol_x && z < vol_z) stb(vres, v, x * (int)sizeof(float), z * row_bytes);
         }
         return;
     }
@@ -384,7 +392,7 @@ __device__ __forceinline__ void bp_tile_body(cptr<BpArgs> A, float2 (&pairs)[CH]
         const bool empty = !(v > kEmpty);
         v = empty ? kSentinel : (positivity ? fmaxf(v, 0.0f) : v);
         if (x < vol_x && z < vol_z && count > 0) {
-            vslice[(size_t)z * vol_x + x] = v;
+            stb(vres, v, x * (int)sizeof(float), z * row_bytes);
             wrote_sent |= empty;
         }
     }

@@ -87,6 +87,35 @@ __device__ __forceinline__ void stg(ATR_GLOBAL_AS bf16_t* p, float v)
     p->bits = (uint16_t)(b >> 16);
 }
 
+// Raw buffer access (a wave-uniform resource over one array, a per-lane byte
+// offset and a scalar byte offset): no VALU for the address.  The resource's
+// range is the array, so a lane offset past it reads 0 instead of faulting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* base, size_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
+}
+template <typename S>
+__device__ __forceinline__ float ldb(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ float ldb<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float ldb<_Float16>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float ldb<bf16_t>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0) << 16);
+}
+__device__ __forceinline__ void stb(__amdgpu_buffer_rsrc_t r, float v, int voff, int soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+
 // Threshold below which a stored value counts as the reference's "empty"
 // (<= -1e10).  FP16 cannot hold -1e10, so its sentinel is the most negative
 // finite half; real data never gets there after weighting by cos/VolZ.
