@@ -407,8 +407,10 @@ int fp_group(int proj, int n_list, int n_waves)
 // meet at a barrier after every z-band, so what a band costs is the largest
 // per-wave load in it; the schedule minimises the sum over bands of that
 // maximum: greedy placement by decreasing cost, then pairwise exchanges.
-// Cost of a pair in a band = the samples its descriptor covers, masked
-// windows weighted 1.4, plus a fixed 8 per non-empty descriptor.
+// Cost of a pair in a band = the samples its descriptor covers, weighted by
+// 1 + |sin theta| (the 64 rays of a tile read LDS 1 / cos theta columns
+// apart, so bank conflicts grow with the tilt) and by 1.4 for masked
+// windows, plus a fixed 8 per non-empty descriptor.
 // tab gets one block of FP_WAVES * FP_MAXP entries per group of `group`
 // tilts: entry [w * FP_MAXP + i] = (local tilt << 16) | tile, or -1.  Only
 // waves 0 .. n_waves - 1 get pairs (the band-prefetching projector keeps its
@@ -427,7 +429,8 @@ void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, 
             const int t = list[j0 + it / ntile], tile = it % ntile;
             for (int b = 0; b < nb; b++) {
                 const int4 d = g.fp_desc[((size_t)t * ntile + tile) * nb + b];
-                double c = d.y > 0 ? d.y * (d.w ? 1.0 : 1.4) + 8.0 : 0.0;
+                const double wt = 1.0 + std::fabs((double)g.cos_sin[2 * t + 1]);
+                double c = d.y > 0 ? d.y * wt * (d.w ? 1.0 : 1.4) + 8.0 : 0.0;
                 cost[(size_t)it * nb + b] = c;
                 total[it] += c;
             }
