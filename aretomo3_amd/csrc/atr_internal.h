@@ -400,6 +400,7 @@ void launch_bp_batch(int storage, const BpItem* items, int n_items, int total, i
 void launch_fp_batch(int storage, const FpItem* items, int n_items, int total, int* counters,
                      size_t shm, int grid, hipStream_t s);
 int batch_blocks_per_cu(int storage, bool fp, size_t fp_shm);
+int fp_batch_blocks_per_cu(int storage, size_t fp_shm);
 void launch_fp_diff(const Plan& p, int subset, hipStream_t s);
 void launch_vol_stats(const Plan& p, hipStream_t s);
 void launch_output(const Plan& p, float* out, hipStream_t s);
