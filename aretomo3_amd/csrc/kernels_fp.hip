@@ -234,32 +234,30 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
     constexpr int RPC = RP > 0 ? RP : 1;
     constexpr int RP_ROWS = RP > 0 ? (FP_BZ_MAX + 1) / RP : 1;
     float pa[RPC][RP_ROWS], pb[RPC][RP_ROWS];
-    auto rp_load = [&](int zb_n, int rows_n, auto full) {
-        constexpr bool FULL = decltype(full)::value;
+    // Every one of the RP_ROWS rows is loaded (a short band repeats its last
+    // row), so the loads' registers are the same on every path: a register
+    // that one path loads and another does not would make the compiler wait
+    // for the loads in flight wherever the paths join.
+    auto rp_load_band = [&](int b_n) {
+        const int zb_n = __builtin_amdgcn_readfirstlane(b_n * bz);
+        const int last = __builtin_amdgcn_readfirstlane(min(bz, vol_z - 1 - zb_n));
         const int zoff = zb_n * vol_x * (int)sizeof(float);
 #pragma unroll
         for (int c = 0; c < RPC; c++) {
             const int col = tid + c * FP_THREADS;
-            // columns 0 .. VolX - 1 (the threads past them load nothing)
-            const int xa = max(col - 1, 0), xb = min(xa + 1, vol_x - 1);
-            if (col >= vol_x) continue;
+            // columns 0 .. VolX - 1 (a thread past them loads column VolX - 1
+            // and writes nothing)
+            const int cl = min(col, vol_x - 1);
+            const int xa = max(cl - 1, 0), xb = min(xa + 1, vol_x - 1);
 #pragma unroll
             for (int r = 0; r < RP_ROWS; r++) {
-                if (FULL || r < rows_n) {
-                    const int so = zoff + r * vol_x * (int)sizeof(float);
-                    pa[c][r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                        vrsrc, xa * (int)sizeof(float), so, 0));
-                    pb[c][r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                        vrsrc, xb * (int)sizeof(float), so, 0));
-                }
+                const int so = zoff + min(r, last) * vol_x * (int)sizeof(float);
+                pa[c][r] = __uint_as_float(
+                    __builtin_amdgcn_raw_buffer_load_b32(vrsrc, xa * (int)sizeof(float), so, 0));
+                pb[c][r] = __uint_as_float(
+                    __builtin_amdgcn_raw_buffer_load_b32(vrsrc, xb * (int)sizeof(float), so, 0));
             }
         }
-    };
-    auto rp_load_band = [&](int b_n) {
-        const int zb_n = __builtin_amdgcn_readfirstlane(b_n * bz);
-        const int rows_n = __builtin_amdgcn_readfirstlane(min(bz, vol_z - 1 - zb_n) + 1);
-        if (rows_n == RP_ROWS) rp_load(zb_n, rows_n, std::true_type{});
-        else rp_load(zb_n, rows_n, std::false_type{});
     };
     auto rp_write = [&](int rows_n, auto full) {
         constexpr bool FULL = decltype(full)::value;
@@ -384,8 +382,7 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
                     const cptr<float4> G = geo + d.x;
                     if (!SENT && d.w != 0) {
                         // the window's samples summed from zero, then added
-                        // to the ray's sum: a fresh loop-carried register
-                        // instead of one lane of the 16-register sum array
+                        // to the ray's sum
                         float part = 0.0f;
                         int unused = 0;
                         fp_run<false, false, ROWB>(bandb, rowb, G, d.y, d.z, tf1[i], xmax1,
