@@ -262,11 +262,9 @@ struct FpArgs {
 // shape: the series' back-projection tile shape, bp_shape(bp_tz, bp_chunk).
 struct BpItem {
     BpArgs a;
-    int item0, n_items_y;      // items of one slice = ztiles * xtiles
+    int item0, n_items_y;      // items of one slice: BP_ITEM_TILES tiles each
     int shape;
 };
-// 0: 16 z rows x 16 tilts per stage, 1: 16 x 8, 2: 8 x 16, 3: 8 x 8
-inline int bp_shape(int tz, int chunk) { return (tz == 8 ? 2 : 0) + (chunk == 8 ? 1 : 0); }
 
 // The entries of one persistent work-queue launch, laid out largest work
 // item first: the items the launch claims last are then the cheapest ones,
@@ -293,10 +291,49 @@ int lay_out_entries(std::vector<QueueEntry<Item>>& v, Item* out)
     }
     return total;
 }
+// A batched back-projection work item is this many consecutive tiles of
+// one slice: a tile is a few microseconds of work, and each item costs the
+// queue's claim and two barriers.
+constexpr int BP_ITEM_TILES = 4;
+inline int bp_items_per_slice(const BpArgs& a)
+{
+    return (a.n_ztiles * a.n_xtiles + BP_ITEM_TILES - 1) / BP_ITEM_TILES;
+}
+// 0: 16 z rows x 16 tilts per stage, 1: 16 x 8, 2: 8 x 16, 3: 8 x 8
+constexpr int BP_SHAPES = 4;
+inline int bp_shape(int tz, int chunk) { return (tz == 8 ? 2 : 0) + (chunk == 8 ? 1 : 0); }
+// Batched back-projection: one persistent launch per tile shape present.
+struct BpLaunch {
+    int first;       // first entry of its queue in the item array
+    int n_entries, total, shape;
+};
+// Writes the queues to out (room for ent.size() entries), each laid out
+// largest item first, and returns their launches in order.
+std::vector<BpLaunch> lay_out_bp(const std::vector<QueueEntry<BpItem>>& ent, BpItem* out);
+
 struct FpItem {
     FpArgs a;
     int item0, n_groups;       // items of one slice = tilt groups
 };
+
+// Batched forward projection: one persistent launch per staging variant
+// present in a set of queue entries (each variant's kernel holds only its
+// own code, as the per-series kernels do), then one launch of the
+// empty-voxel variant over every entry, whose blocks leave at once on slices
+// without empty voxels.  Variants 0-3: band prefetch by DMA, 4-7: direct
+// staging (VolX 512, 1024, 2048, any), 8-9: register prefetch (VolX 1024,
+// 2048).
+constexpr int FP_VARIANT_EMPTY = 10;
+constexpr int FP_VARIANTS = 11;
+int fp_variant(const FpArgs& a);
+struct FpLaunch {
+    int first;       // first entry of its queue in the item array
+    int n_entries, total, variant;
+    size_t shm;      // dynamic LDS bytes
+};
+// Writes the queues to out (room for 2 ent.size() entries), each laid out
+// largest item first, and returns their launches in order.
+std::vector<FpLaunch> lay_out_fp(const std::vector<QueueEntry<FpItem>>& ent, FpItem* out);
 
 // Re-projector launch arguments (passed by value): tilts [first, first +
 // n_range) back-projected with host FP64 cos / sin / cos(a - t) tables, then
@@ -395,12 +432,16 @@ FpArgs fp_args(const Plan& p, const int* fp_list, int n_list, const int* sched);
 size_t fp_lds_bytes(int bz, int vol_x, int pf = 0);
 // step 0: initial back-projection, 1 + s: subset s (arguments in p.d_bp_args)
 void launch_bp(const Plan& p, int step, hipStream_t s);
-void launch_bp_batch(int storage, const BpItem* items, int n_items, int total, int* counter,
-                     int grid, hipStream_t s);
-void launch_fp_batch(int storage, const FpItem* items, int n_items, int total, int* counters,
-                     size_t shm, int grid, hipStream_t s);
-int batch_blocks_per_cu(int storage, bool fp, size_t fp_shm);
-int fp_batch_blocks_per_cu(int storage, size_t fp_shm);
+// items: the launch's queue (the array lay_out_bp wrote, at l.first)
+void launch_bp_batch(int storage, const BpLaunch& l, const BpItem* items, int* counter, int grid,
+                     hipStream_t s);
+// persistent grid of a launch: CUs x resident blocks per CU
+int bp_batch_grid(int storage, const BpLaunch& l);
+// items: the launch's queue (the array lay_out_fp wrote, at l.first)
+void launch_fp_batch(int storage, const FpLaunch& l, const FpItem* items, int* counter, int grid,
+                     hipStream_t s);
+// persistent grid of a launch: CUs x resident blocks per CU
+int fp_batch_grid(int storage, const FpLaunch& l);
 void launch_fp_diff(const Plan& p, int subset, hipStream_t s);
 void launch_vol_stats(const Plan& p, hipStream_t s);
 void launch_output(const Plan& p, float* out, hipStream_t s);

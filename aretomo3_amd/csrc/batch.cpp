@@ -23,14 +23,15 @@ struct atr_batch {
         int n_entries;
         int total;      // queue items
         int counter;    // queue counter slot
+        int grid;       // persistent grid
+        atr::FpLaunch fl;   // forward projection: the launch (variant, LDS)
+        atr::BpLaunch bl;   // back-projection: the launch (tile shape)
     };
     std::vector<Step> steps;
     atr::BpItem* d_bp = nullptr;
     atr::FpItem* d_fp = nullptr;
     int* d_counters = nullptr;
     int n_counters = 0;
-    size_t fp_shm = 0;
-    int bp_grid = 1, fp_grid = 1;
     bool graph = false;
     hipStream_t stream = nullptr;   // the batch's own stream (graph capture needs one)
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -63,11 +64,20 @@ static atr_status batch_build(atr_batch* b)
     for (atr_plan* h : b->plans) {
         max_iters = std::max(max_iters, h->p.iters);
         max_sub = std::max(max_sub, h->p.n_subsets);
-        b->fp_shm = std::max(b->fp_shm, fp_lds_bytes(h->p.geo.fp_bz, h->p.vol_x, h->p.fp_pf));
     }
+    // one back-projection launch per tile shape present
+    auto push_bp = [&](std::vector<QueueEntry<BpItem>>& ent) {
+        const int b_first = (int)bp.size();
+        bp.resize(b_first + ent.size());
+        for (const BpLaunch& l : lay_out_bp(ent, bp.data() + b_first)) {
+            atr_batch::Step st{false, b_first + l.first, l.n_entries, l.total, b->n_counters++,
+                               0, {}, l};
+            st.grid = std::min(bp_batch_grid(b->storage, l), std::max(l.total, 1));
+            b->steps.push_back(st);
+        }
+    };
     // initial back-projection of every series (relax 1, CTomoSart.cpp:68-69)
     {
-        atr_batch::Step st{false, (int)bp.size(), 0, 0, b->n_counters++};
         std::vector<QueueEntry<BpItem>> ent;
         for (atr_plan* h : b->plans) {
             const Plan& p = h->p;
@@ -75,19 +85,14 @@ static atr_status batch_build(atr_batch* b)
             if (n_all == 0) continue;
             BpItem e{};
             e.a = bp_args(p, p.d_sino, p.n, p.d_lists, n_all, 1.0f, true);
-            e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
+            e.n_items_y = bp_items_per_slice(e.a);
             e.shape = bp_shape(p.bp_tz, p.bp_chunk);
             ent.push_back({e, p.vol_y * e.n_items_y, (double)n_all * p.bp_tz});
-            st.n_entries++;
         }
-        bp.resize(st.first + ent.size());
-        st.total = lay_out_entries(ent, bp.data() + st.first);
-        if (st.n_entries) b->steps.push_back(st);
+        push_bp(ent);
     }
     for (int it = 0; it < max_iters; it++)
         for (int sb = 0; sb < max_sub; sb++) {
-            atr_batch::Step sf{true, (int)fp.size(), 0, 0, b->n_counters++};
-            atr_batch::Step sp{false, (int)bp.size(), 0, 0, b->n_counters++};
             std::vector<QueueEntry<FpItem>> fent;
             std::vector<QueueEntry<BpItem>> bent;
             for (atr_plan* h : b->plans) {
@@ -101,23 +106,25 @@ static atr_status batch_build(atr_batch* b)
                 f.n_groups = (f1 - f0 + f.a.group - 1) / f.a.group;
                 fent.push_back({f, p.vol_y * f.n_groups,
                                 (double)p.vol_x * p.vol_z * std::min(f.a.group, f1 - f0)});
-                sf.n_entries++;
                 if (b1 > b0) {
                     BpItem e{};
                     e.a = bp_args(p, p.d_diff, p.max_subset, p.d_lists + n_all + b0, b1 - b0,
                                   p.relax, false);
-                    e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
+                    e.n_items_y = bp_items_per_slice(e.a);
                     e.shape = bp_shape(p.bp_tz, p.bp_chunk);
                     bent.push_back({e, p.vol_y * e.n_items_y, (double)(b1 - b0) * p.bp_tz});
-                    sp.n_entries++;
                 }
             }
-            fp.resize(sf.first + fent.size());
-            sf.total = lay_out_entries(fent, fp.data() + sf.first);
-            bp.resize(sp.first + bent.size());
-            sp.total = lay_out_entries(bent, bp.data() + sp.first);
-            if (sf.n_entries) b->steps.push_back(sf);
-            if (sp.n_entries) b->steps.push_back(sp);
+            // one forward-projection launch per staging variant present
+            const int f_first = (int)fp.size();
+            fp.resize(f_first + 2 * fent.size());
+            for (const FpLaunch& l : lay_out_fp(fent, fp.data() + f_first)) {
+                atr_batch::Step sf{true, f_first + l.first, l.n_entries, l.total,
+                                   b->n_counters++, 0, l, {}};
+                sf.grid = std::min(fp_batch_grid(b->storage, l), std::max(l.total, 1));
+                b->steps.push_back(sf);
+            }
+            push_bp(bent);
         }
     if (!bp.empty()) ATR_HIP(hipMalloc((void**)&b->d_bp, bp.size() * sizeof(BpItem)));
     if (!fp.empty()) ATR_HIP(hipMalloc((void**)&b->d_fp, fp.size() * sizeof(FpItem)));
@@ -126,11 +133,6 @@ static atr_status batch_build(atr_batch* b)
         ATR_HIP(hipMemcpy(b->d_bp, bp.data(), bp.size() * sizeof(BpItem), hipMemcpyHostToDevice));
     if (!fp.empty())
         ATR_HIP(hipMemcpy(b->d_fp, fp.data(), fp.size() * sizeof(FpItem), hipMemcpyHostToDevice));
-    int dev = 0, cus = 1;
-    ATR_HIP(hipGetDevice(&dev));
-    ATR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    b->bp_grid = cus * batch_blocks_per_cu(b->storage, false, 0);
-    b->fp_grid = cus * batch_blocks_per_cu(b->storage, true, b->fp_shm);
     ATR_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     ATR_HIP(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming));
     ATR_HIP(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming));
@@ -150,13 +152,11 @@ static void batch_steps(atr_batch* b, hipStream_t s, bool timing)
             (void)hipEventRecord(e0, s);
         }
         if (st.fp)
-            launch_fp_batch(b->storage, b->d_fp + st.first, st.n_entries, st.total,
-                            b->d_counters + st.counter, b->fp_shm,
-                            std::min(b->fp_grid, std::max(st.total, 1)), s);
+            launch_fp_batch(b->storage, st.fl, b->d_fp + st.first, b->d_counters + st.counter,
+                            st.grid, s);
         else
-            launch_bp_batch(b->storage, b->d_bp + st.first, st.n_entries, st.total,
-                            b->d_counters + st.counter, std::min(b->bp_grid, std::max(st.total, 1)),
-                            s);
+            launch_bp_batch(b->storage, st.bl, b->d_bp + st.first, b->d_counters + st.counter,
+                            st.grid, s);
         if (timing) {
             (void)hipEventRecord(e1, s);
             float ms = 0.0f;

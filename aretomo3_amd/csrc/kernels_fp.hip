@@ -439,12 +439,26 @@ __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff(const FpArgs* __re
     fp_block<S, SENT, ROWB, PF, RP>(as_cptr(a), band, blockIdx.x, blockIdx.y);
 }
 
-// Forward projection + difference of many series in one persistent launch:
-// blocks take (series, slice, tilt group) items from a queue.  A slice with
-// empty voxels runs the sentinel variant, the others the variant with the
-// band pitch fixed at compile time for VolX 512 / 1024 / 2048 (as the
-// per-series launch does), so one launch covers every slice.
-template <typename S>
+// Forward projection + difference of many series in persistent launches:
+// blocks take (series, slice, tilt group) items from a queue.  One launch
+// runs one staging variant (the staging mode and the band pitch fixed at
+// compile time for VolX 512 / 1024 / 2048, as the per-series launch does),
+// so each variant's kernel holds only its own code and registers; variant
+// FP_VARIANT_EMPTY is the empty-voxel projector.  Every block of a work item
+// whose slice does not match the variant's empty-voxel flag leaves at once.
+template <typename S, int V>
+__device__ __forceinline__ void fp_variant_block(cptr<FpArgs> A, float2* band, int gx, int y)
+{
+    // VolX class: 0: 512, 1: 1024, 2: 2048, 3: any (pitch at run time)
+    constexpr int W = V < 8 ? V % 4 : V - 7;
+    constexpr int ROWB = W == 0 ? 513 * 8 : W == 1 ? 1025 * 8 : W == 2 ? 2049 * 8 : 0;
+    if constexpr (V == FP_VARIANT_EMPTY) fp_block<S, true>(A, band, gx, y);
+    else if constexpr (V < 4) fp_block<S, false, ROWB, true>(A, band, gx, y);
+    else if constexpr (V < 8) fp_block<S, false, ROWB>(A, band, gx, y);
+    else fp_block<S, false, ROWB, false, W>(A, band, gx, y);   // RP: 1 or 2 columns
+}
+
+template <typename S, int V>
 __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff_batch(const FpItem* __restrict__ e,
                                                                    int n_e, int total,
                                                                    int* __restrict__ counter)
@@ -458,36 +472,7 @@ __global__ __launch_bounds__(FP_THREADS) void k_for_proj_diff_batch(const FpItem
         const int local = it - E[s].item0;
         const int ng = E[s].n_groups;
         const int y = local / ng, gx = local - y * ng;
-        const cptr<FpArgs> A = &E[s].a;
-        if (as_cptr(A->vflag)[y] != 0) {
-            fp_block<S, true>(A, band, gx, y);
-        } else {
-            // the staging mode of the series' own plan (its band rows and
-            // wave schedule were built for it), so a series computes here
-            // exactly what it computes alone
-            if (A->pf == FP_PF_DMA) {
-                switch (A->vol_x) {
-                case 512: fp_block<S, false, 513 * 8, true>(A, band, gx, y); break;
-                case 1024: fp_block<S, false, 1025 * 8, true>(A, band, gx, y); break;
-                case 2048: fp_block<S, false, 2049 * 8, true>(A, band, gx, y); break;
-                default: fp_block<S, false, 0, true>(A, band, gx, y); break;
-                }
-            } else {
-                const bool rp = A->pf == FP_PF_REG;
-                switch (A->vol_x) {
-                case 512: fp_block<S, false, 513 * 8>(A, band, gx, y); break;
-                case 1024:
-                    if (rp) fp_block<S, false, 1025 * 8, false, 1>(A, band, gx, y);
-                    else fp_block<S, false, 1025 * 8>(A, band, gx, y);
-                    break;
-                case 2048:
-                    if (rp) fp_block<S, false, 2049 * 8, false, 2>(A, band, gx, y);
-                    else fp_block<S, false, 2049 * 8>(A, band, gx, y);
-                    break;
-                default: fp_block<S, false, 0>(A, band, gx, y); break;
-                }
-            }
-        }
+        fp_variant_block<S, V>(&E[s].a, band, gx, y);
         it = publish_item(ahead);
     }
 }
@@ -600,47 +585,87 @@ void launch_fp_diff(const Plan& p, int subset, hipStream_t s)
     else launch_fp_t<float>(p, subset, s);
 }
 
-template <typename S>
-static void fp_batch_attrs()
+// Staging variant of a series' plan (slices without empty voxels).
+int fp_variant(const FpArgs& a)
+{
+    const int w = a.vol_x == 512 ? 0 : a.vol_x == 1024 ? 1 : a.vol_x == 2048 ? 2 : 3;
+    if (a.pf == FP_PF_DMA) return w;
+    if (a.pf == FP_PF_REG && (w == 1 || w == 2)) return 7 + w;
+    return 4 + w;
+}
+
+std::vector<FpLaunch> lay_out_fp(const std::vector<QueueEntry<FpItem>>& ent, FpItem* out)
+{
+    std::vector<FpLaunch> launches;
+    int first = 0;
+    for (int v = 0; v < FP_VARIANTS; v++) {
+        std::vector<QueueEntry<FpItem>> g;
+        size_t shm = 0;
+        for (const auto& e : ent) {
+            const FpArgs& a = e.item.a;
+            if (v != FP_VARIANT_EMPTY && fp_variant(a) != v) continue;
+            g.push_back(e);
+            shm = std::max(shm, fp_lds_bytes(a.bz, a.vol_x, v == FP_VARIANT_EMPTY ? 0 : a.pf));
+        }
+        if (g.empty()) continue;
+        const int total = lay_out_entries(g, out + first);
+        launches.push_back(FpLaunch{first, (int)g.size(), total, v, shm});
+        first += (int)g.size();
+    }
+    return launches;
+}
+
+template <typename S, int V>
+static int fp_batch_v(const FpLaunch* l, const FpItem* items, int* counter, int grid,
+                      hipStream_t s)
 {
     static std::atomic<unsigned long long> done{0};
-    allow_big_lds(k_for_proj_diff_batch<S>, done, FP_LDS_PF);   // the largest mode's LDS
+    // the DMA variants' LDS holds the raw rows too (and the queue's word)
+    allow_big_lds(k_for_proj_diff_batch<S, V>, done, V < 4 ? FP_LDS_PF : FP_LDS_BUDGET);
+    if (!items) {
+        // resident blocks per CU at this LDS size
+        int nb = 0;
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, k_for_proj_diff_batch<S, V>, FP_THREADS, l->shm);
+        return (e == hipSuccess && nb > 0) ? nb : 1;
+    }
+    hipLaunchKernelGGL((k_for_proj_diff_batch<S, V>), dim3(grid), dim3(FP_THREADS), l->shm, s,
+                       items, l->n_entries, l->total, counter);
+    return 0;
 }
 
-template <typename S>
-static void launch_fp_batch_t(const FpItem* items, int n_items, int total, int* counters,
-                              size_t shm, int grid, hipStream_t s)
+template <typename S, int V = 0>
+static int fp_batch_s(const FpLaunch* l, const FpItem* items, int* counter, int grid,
+                      hipStream_t s)
 {
-    fp_batch_attrs<S>();
-    hipLaunchKernelGGL((k_for_proj_diff_batch<S>), dim3(grid), dim3(FP_THREADS), shm, s,
-                       items, n_items, total, counters);
+    if constexpr (V < FP_VARIANTS) {
+        if (l->variant == V) return fp_batch_v<S, V>(l, items, counter, grid, s);
+        return fp_batch_s<S, V + 1>(l, items, counter, grid, s);
+    }
+    return 1;
 }
 
-void launch_fp_batch(int storage, const FpItem* items, int n_items, int total, int* counters,
-                     size_t shm, int grid, hipStream_t s)
+static int fp_batch(int storage, const FpLaunch* l, const FpItem* items, int* counter, int grid,
+                    hipStream_t s)
 {
-    if (storage == ATR_STORAGE_F16) launch_fp_batch_t<_Float16>(items, n_items, total, counters, shm, grid, s);
-    else if (storage == ATR_STORAGE_BF16) launch_fp_batch_t<bf16_t>(items, n_items, total, counters, shm, grid, s);
-    else launch_fp_batch_t<float>(items, n_items, total, counters, shm, grid, s);
+    if (storage == ATR_STORAGE_F16) return fp_batch_s<_Float16>(l, items, counter, grid, s);
+    if (storage == ATR_STORAGE_BF16) return fp_batch_s<bf16_t>(l, items, counter, grid, s);
+    return fp_batch_s<float>(l, items, counter, grid, s);
 }
 
-// Resident blocks per CU of the batched forward projection (the persistent
-// grid size).
-template <typename S>
-static int fp_batch_blocks_t(size_t fp_shm)
+void launch_fp_batch(int storage, const FpLaunch& l, const FpItem* items, int* counter, int grid,
+                     hipStream_t s)
 {
-    fp_batch_attrs<S>();
-    int nb = 0;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &nb, k_for_proj_diff_batch<S>, FP_THREADS, fp_shm);
-    return (e == hipSuccess && nb > 0) ? nb : 1;
+    (void)fp_batch(storage, &l, items, counter, std::max(1, std::min(grid, l.total)), s);
 }
 
-int fp_batch_blocks_per_cu(int storage, size_t fp_shm)
+int fp_batch_grid(int storage, const FpLaunch& l)
 {
-    if (storage == ATR_STORAGE_F16) return fp_batch_blocks_t<_Float16>(fp_shm);
-    if (storage == ATR_STORAGE_BF16) return fp_batch_blocks_t<bf16_t>(fp_shm);
-    return fp_batch_blocks_t<float>(fp_shm);
+    int dev = 0, cus = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 1;
+    return cus * fp_batch(storage, &l, nullptr, nullptr, 0, nullptr);
 }
 
 }  // namespace atr

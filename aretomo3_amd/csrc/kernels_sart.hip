@@ -425,20 +425,17 @@ __global__ __launch_bounds__(BP_TX) void k_wbp_grid_bp(const BpArgs* __restrict_
     bp_tile<S, BP_TZ, BP_CHUNK, true>(as_cptr(a), pairs, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-template <int TZ, int CH>
-__device__ __forceinline__ float2 (&bp_pairs(float2* lds))[CH][bp_win(TZ)]
-{
-    return *reinterpret_cast<float2(*)[CH][bp_win(TZ)]>(lds);
-}
-
-// Back-projection of many series in one persistent launch: blocks take
-// (series, slice, z-tile, x-tile) items from a queue; entries are ordered by
-// their first item.  Each entry runs its own series' tile shape.
-template <typename S>
+// Back-projection of many series in persistent launches: blocks take
+// (series, slice, run of BP_ITEM_TILES tiles) items from a queue; entries
+// are ordered by their first item.  One launch runs one tile shape (0: 16 z rows x 16 tilts
+// per stage, 1: 16 x 8, 2: 8 x 16, 3: 8 x 8), so each shape's kernel holds
+// only its own code and registers, as the per-series kernels do.
+template <typename S, int SHAPE>
 __global__ __launch_bounds__(BP_TX) void k_back_proj_batch(const BpItem* __restrict__ e, int n_e,
                                                            int total, int* __restrict__ counter)
 {
-    __shared__ float2 lds[BP_CHUNK * bp_win(BP_TZ)];   // the largest shape's stage
+    constexpr int TZ = (SHAPE & 2) ? 8 : 16, CH = (SHAPE & 1) ? 8 : 16;
+    __shared__ float2 pairs[CH][bp_win(TZ)];
     const cptr<BpItem> E = as_cptr(e);
     int s = 0;
     for (int it = publish_item(claim_item(counter)); it < total;) {
@@ -446,14 +443,13 @@ __global__ __launch_bounds__(BP_TX) void k_back_proj_batch(const BpItem* __restr
         s = entry_of(E, n_e, it, s);
         const int local = it - E[s].item0;
         const int per_y = E[s].n_items_y;
-        const int y = local / per_y, r = local - y * per_y;
-        const int zt = r / E[s].a.n_xtiles, xt = r - zt * E[s].a.n_xtiles;
-        const cptr<BpArgs> A = &E[s].a;
-        switch (E[s].shape) {
-        case 1: bp_tile<S, 16, 8>(A, bp_pairs<16, 8>(lds), xt, zt, y); break;
-        case 2: bp_tile<S, 8, 16>(A, bp_pairs<8, 16>(lds), xt, zt, y); break;
-        case 3: bp_tile<S, 8, 8>(A, bp_pairs<8, 8>(lds), xt, zt, y); break;
-        default: bp_tile<S, 16, 16>(A, bp_pairs<16, 16>(lds), xt, zt, y); break;
+        const int y = local / per_y, q = local - y * per_y;
+        const int nx = E[s].a.n_xtiles, nt = E[s].a.n_ztiles * nx;
+        for (int g = 0; g < BP_ITEM_TILES; g++) {
+            const int r = q * BP_ITEM_TILES + g;
+            if (r >= nt) break;
+            const int zt = r / nx, xt = r - zt * nx;
+            bp_tile<S, TZ, CH>(&E[s].a, pairs, xt, zt, y);
         }
         it = publish_item(ahead);
     }
@@ -683,33 +679,71 @@ void launch_wbp_bp(const Plan& p, int slab, int ny, hipStream_t s)
     else hipLaunchKernelGGL(k_wbp_grid_bp<float>, grid, dim3(BP_TX), 0, s, a);
 }
 
-void launch_bp_batch(int storage, const BpItem* items, int n_items, int total, int* counter,
-                     int grid, hipStream_t s)
+std::vector<BpLaunch> lay_out_bp(const std::vector<QueueEntry<BpItem>>& ent, BpItem* out)
 {
-    if (storage == ATR_STORAGE_F16)
-        hipLaunchKernelGGL(k_back_proj_batch<_Float16>, dim3(grid), dim3(BP_TX), 0, s, items, n_items, total, counter);
-    else if (storage == ATR_STORAGE_BF16)
-        hipLaunchKernelGGL(k_back_proj_batch<bf16_t>, dim3(grid), dim3(BP_TX), 0, s, items, n_items, total, counter);
-    else
-        hipLaunchKernelGGL(k_back_proj_batch<float>, dim3(grid), dim3(BP_TX), 0, s, items, n_items, total, counter);
+    std::vector<BpLaunch> launches;
+    int first = 0;
+    for (int shape = 0; shape < BP_SHAPES; shape++) {
+        std::vector<QueueEntry<BpItem>> g;
+        for (const auto& e : ent)
+            if (e.item.shape == shape) g.push_back(e);
+        if (g.empty()) continue;
+        const int total = lay_out_entries(g, out + first);
+        launches.push_back(BpLaunch{first, (int)g.size(), total, shape});
+        first += (int)g.size();
+    }
+    return launches;
 }
 
-// Resident blocks per CU of the batched kernels (the persistent grid size).
+// items == nullptr: resident blocks per CU, else the launch
+template <typename S, int SHAPE>
+static int bp_batch_v(const BpLaunch* l, const BpItem* items, int* counter, int grid,
+                      hipStream_t s)
+{
+    if (!items) {
+        int nb = 0;
+        const hipError_t e =
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_back_proj_batch<S, SHAPE>, BP_TX, 0);
+        return (e == hipSuccess && nb > 0) ? nb : 1;
+    }
+    hipLaunchKernelGGL((k_back_proj_batch<S, SHAPE>), dim3(grid), dim3(BP_TX), 0, s, items,
+                       l->n_entries, l->total, counter);
+    return 0;
+}
+
 template <typename S>
-static int bp_batch_blocks_t()
+static int bp_batch_s(const BpLaunch* l, const BpItem* items, int* counter, int grid,
+                      hipStream_t s)
 {
-    int nb = 0;
-    const hipError_t e =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_back_proj_batch<S>, BP_TX, 0);
-    return (e == hipSuccess && nb > 0) ? nb : 1;
+    switch (l->shape) {
+    case 1: return bp_batch_v<S, 1>(l, items, counter, grid, s);
+    case 2: return bp_batch_v<S, 2>(l, items, counter, grid, s);
+    case 3: return bp_batch_v<S, 3>(l, items, counter, grid, s);
+    default: return bp_batch_v<S, 0>(l, items, counter, grid, s);
+    }
 }
 
-int batch_blocks_per_cu(int storage, bool fp, size_t fp_shm)
+static int bp_batch(int storage, const BpLaunch* l, const BpItem* items, int* counter, int grid,
+                    hipStream_t s)
 {
-    if (fp) return fp_batch_blocks_per_cu(storage, fp_shm);
-    if (storage == ATR_STORAGE_F16) return bp_batch_blocks_t<_Float16>();
-    if (storage == ATR_STORAGE_BF16) return bp_batch_blocks_t<bf16_t>();
-    return bp_batch_blocks_t<float>();
+    if (storage == ATR_STORAGE_F16) return bp_batch_s<_Float16>(l, items, counter, grid, s);
+    if (storage == ATR_STORAGE_BF16) return bp_batch_s<bf16_t>(l, items, counter, grid, s);
+    return bp_batch_s<float>(l, items, counter, grid, s);
+}
+
+void launch_bp_batch(int storage, const BpLaunch& l, const BpItem* items, int* counter, int grid,
+                     hipStream_t s)
+{
+    (void)bp_batch(storage, &l, items, counter, std::max(1, std::min(grid, l.total)), s);
+}
+
+int bp_batch_grid(int storage, const BpLaunch& l)
+{
+    int dev = 0, cus = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 1;
+    return cus * bp_batch(storage, &l, nullptr, nullptr, 0, nullptr);
 }
 
 void launch_zero(float* ptr, size_t n, hipStream_t s)

@@ -40,6 +40,9 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr int kRing = 4;   // rounds whose item arrays may be in flight at once
+// queue counters of a round: one per forward-projection variant, then one
+// per back-projection tile shape
+constexpr int kCounters = atr::FP_VARIANTS + atr::BP_SHAPES;
 constexpr int kStarveRounds = 16;   // shortest-first: longest wait of a series, in rounds
 
 // Device arena: chunks from hipMalloc, kept until the arena is destroyed;
@@ -191,7 +194,7 @@ struct atr_server {
     hipEvent_t ev_slot[kRing] = {};
     bool slot_used[kRing] = {};
     int slot = 0;
-    int bp_grid = 1, fp_grid = 1, storage = 0;
+    int storage = 0;
     // statistics
     uint64_t rounds = 0, max_batch = 0;
 };
@@ -354,9 +357,11 @@ static atr_status grow_slots(atr_server* s, int need)
         s->h_bp[k] = nullptr;
         s->h_fp[k] = nullptr;
         ATR_HIP(hipMalloc((void**)&s->d_bp[k], sizeof(BpItem) * cap));
-        ATR_HIP(hipMalloc((void**)&s->d_fp[k], sizeof(FpItem) * cap));
+        // forward projection: each entry in its variant's queue and in the
+        // empty-voxel queue
+        ATR_HIP(hipMalloc((void**)&s->d_fp[k], sizeof(FpItem) * 2 * cap));
         ATR_HIP(hipHostMalloc((void**)&s->h_bp[k], sizeof(BpItem) * cap, hipHostMallocDefault));
-        ATR_HIP(hipHostMalloc((void**)&s->h_fp[k], sizeof(FpItem) * cap, hipHostMallocDefault));
+        ATR_HIP(hipHostMalloc((void**)&s->h_fp[k], sizeof(FpItem) * 2 * cap, hipHostMallocDefault));
         s->slot_used[k] = false;
     }
     s->cap = cap;
@@ -414,7 +419,6 @@ static atr_status run_round(atr_server* s)
         }
     }
     int n_take = 0;
-    size_t shm = 0;
     std::vector<QueueEntry<FpItem>> fent;
     std::vector<QueueEntry<BpItem>> bent;
     for (size_t i = 0; i < act.size(); i++) {
@@ -426,7 +430,7 @@ static atr_status run_round(atr_server* s)
         if (r.step == 0) {
             BpItem e{};
             e.a = bp_args(p, p.d_sino, p.n, p.d_lists, n_all, 1.0f, true);
-            e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
+            e.n_items_y = bp_items_per_slice(e.a);
             e.shape = bp_shape(p.bp_tz, p.bp_chunk);
             bent.push_back({e, p.vol_y * e.n_items_y, (double)n_all * p.bp_tz});
         } else if (r.step <= r.last) {
@@ -438,33 +442,37 @@ static atr_status run_round(atr_server* s)
             f.n_groups = (f1 - f0 + f.a.group - 1) / f.a.group;
             fent.push_back({f, p.vol_y * f.n_groups,
                             (double)p.vol_x * p.vol_z * std::min(f.a.group, f1 - f0)});
-            shm = std::max(shm, fp_lds_bytes(p.geo.fp_bz, p.vol_x, p.fp_pf));
             if (b1 > b0) {
                 BpItem e{};
                 e.a = bp_args(p, p.d_diff, p.max_subset, p.d_lists + n_all + b0, b1 - b0, p.relax,
                               false);
-                e.n_items_y = e.a.n_ztiles * e.a.n_xtiles;
+                e.n_items_y = bp_items_per_slice(e.a);
                 e.shape = bp_shape(p.bp_tz, p.bp_chunk);
                 bent.push_back({e, p.vol_y * e.n_items_y, (double)(b1 - b0) * p.bp_tz});
             }
         }
     }
-    const int n_fp = (int)fent.size(), n_bp = (int)bent.size();
-    const int tot_fp = lay_out_entries(fent, s->h_fp[k]);
-    const int tot_bp = lay_out_entries(bent, s->h_bp[k]);
+    // one forward-projection launch per staging variant present, one
+    // back-projection launch per tile shape
+    const std::vector<FpLaunch> fl = lay_out_fp(fent, s->h_fp[k]);
+    const int n_fp = fl.empty() ? 0 : fl.back().first + fl.back().n_entries;
+    const std::vector<BpLaunch> bl = lay_out_bp(bent, s->h_bp[k]);
+    const int n_bp = (int)bent.size();
     hipStream_t q = s->s_comp;
-    launch_zero_int(s->d_cnt[k], 3, q);
+    launch_zero_int(s->d_cnt[k], kCounters, q);
     if (n_fp) {
         ATR_HIP(hipMemcpyAsync(s->d_fp[k], s->h_fp[k], sizeof(FpItem) * n_fp,
                                hipMemcpyHostToDevice, q));
-        launch_fp_batch(s->storage, s->d_fp[k], n_fp, tot_fp, s->d_cnt[k], shm,
-                        std::min(s->fp_grid, tot_fp), q);
+        for (size_t i = 0; i < fl.size(); i++)
+            launch_fp_batch(s->storage, fl[i], s->d_fp[k] + fl[i].first, s->d_cnt[k] + i,
+                            fp_batch_grid(s->storage, fl[i]), q);
     }
     if (n_bp) {
         ATR_HIP(hipMemcpyAsync(s->d_bp[k], s->h_bp[k], sizeof(BpItem) * n_bp,
                                hipMemcpyHostToDevice, q));
-        launch_bp_batch(s->storage, s->d_bp[k], n_bp, tot_bp, s->d_cnt[k] + 2,
-                        std::min(s->bp_grid, tot_bp), q);
+        for (size_t i = 0; i < bl.size(); i++)
+            launch_bp_batch(s->storage, bl[i], s->d_bp[k] + bl[i].first,
+                            s->d_cnt[k] + FP_VARIANTS + i, bp_batch_grid(s->storage, bl[i]), q);
     }
     ATR_HIP(hipEventRecord(s->ev_slot[k], q));
     s->slot_used[k] = true;
@@ -719,15 +727,11 @@ atr_status atr_server_create(const atr_server_config* cfg, atr_server** out)
         if (hipStreamCreateWithFlags(q, hipStreamNonBlocking) != hipSuccess)
             return bail(fail(ATR_ERR_HIP, "hipStreamCreate"));
     for (int k = 0; k < kRing; k++) {
-        if (hipMalloc((void**)&s->d_cnt[k], 4 * sizeof(int)) != hipSuccess ||
+        if (hipMalloc((void**)&s->d_cnt[k], kCounters * sizeof(int)) != hipSuccess ||
             hipEventCreateWithFlags(&s->ev_slot[k], hipEventDisableTiming) != hipSuccess)
             return bail(fail(ATR_ERR_OUT_OF_MEMORY, "atr_server_create: ring"));
     }
     if (grow_slots(s, std::max(8, cfg->max_active)) != ATR_OK) return bail(ATR_ERR_OUT_OF_MEMORY);
-    int cus = 1;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device);
-    s->bp_grid = cus * batch_blocks_per_cu(s->storage, false, 0);
-    s->fp_grid = cus * batch_blocks_per_cu(s->storage, true, FP_LDS_BUDGET);
     s->th_prep = std::thread(prep_main, s);
     s->th_sched = std::thread(sched_main, s);
     s->th_fin = std::thread(fin_main, s);

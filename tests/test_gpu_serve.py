@@ -43,10 +43,10 @@ def alone(raw, tilts, vz, kw, method, storage):
     return out
 
 
-# a longer series (61 tilts, 40 x 5 SART steps: ~200 rounds) that is still
+# a longer series (61 tilts, 160 x 5 SART steps: ~800 rounds) that is still
 # running when the small requests behind it have been prepared (a few ms of
-# host work), so they join its rounds
-LEAD = (200, 256, 64, 2.0, 96, 40, 5, 86.0, True)
+# host work each), so they join its rounds
+LEAD = (200, 256, 64, 2.0, 96, 160, 5, 86.0, True)
 
 
 @pytest.mark.parametrize("schedule", [L.SCHEDULE_FAIR_SHARE, L.SCHEDULE_EVERY_STEP,
