@@ -217,8 +217,8 @@ class Plan:
 
 class Batch:
     """Several SART plans reconstructed together (atr_batch_*): every
-    back-projection / forward-projection step of all series is one persistent
-    launch; graph=True captures the iteration sequence in a HIP graph once and
+    back-projection / forward-projection step of all series is a few persistent
+    launches (one per staging variant and tile shape); graph=True captures the iteration sequence in a HIP graph once and
     replays it.  The plans must outlive the batch."""
 
     def __init__(self, plans, graph=False):

@@ -470,8 +470,9 @@ def batch_bench(args):
     {128,256,384}, tilt axis uniform 80-95 deg + shifts +-3 px, 200-800
     ellipsoids, noise 10%), FP32 projection storage.  A step is every series'
     WBP, then the SART of all 16 (15 iterations x 5 interleaved subsets, relax
-    0.3, positivity off) as one persistent launch per SART pass, the passes
-    replayed from a HIP graph; forward-projection band rows per series from
+    0.3, positivity off) as persistent launches per SART pass (one per staging
+    variant and tile shape among the series), the passes replayed from a HIP
+    graph; forward-projection band rows per series from
     the autotuner.  Reported beside the same work run series by series."""
     from aretomo3_amd import Batch, DeviceArray, Plan, synchronize
     from aretomo3_amd import _lib as L

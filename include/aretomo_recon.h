@@ -213,8 +213,10 @@ atr_status atr_phantom_tilt_series(uint64_t seed, int proj_x, int proj_y, int n_
  * No single reference counterpart: the reference reconstructs one series at a
  * time (CDoSartRecon::DoIt, AreTomo/Recon/CDoSartRecon.cpp:30-57, called per
  * series from AreTomo/CAreTomoMain.cpp:809-848).  A batch runs every
- * back-projection and forward-projection step of all its series as one
- * persistent work-queue launch; with ATR_BATCH_GRAPH the iteration sequence
+ * back-projection and forward-projection step of all its series as a few
+ * persistent work-queue launches (one per forward-projection staging variant
+ * and per back-projection tile shape among the series, plus the empty-voxel
+ * forward projector); with ATR_BATCH_GRAPH the iteration sequence
  * is captured once into a HIP graph and replayed.  Each series gives the
  * same volume as atr_execute on its plan alone.  Plans must be SART plans
  * with one storage type, must outlive the batch, and must not be executed
@@ -238,9 +240,8 @@ void atr_batch_destroy(atr_batch* batch);
  * reconstruction of the same series) in arrival order while their device
  * bytes fit the HBM budget, keeps each series' data on the GPU from
  * pre-stage to output, and advances all admitted series together: every
- * round is one persistent forward-projection and one persistent
- * back-projection launch carrying one SART step of each series whose turn
- * it is (every series, or by fair share: ATR_SCHEDULE_*).  Requests
+ * round is the batched launches of one SART step (as in a batch) carrying
+ * one SART step of each series whose turn it is (ATR_SCHEDULE_*).  Requests
  * admitted mid-run join at the next round.  Plans are built on a host thread
  * of the server ahead of admission; input uploads, WBP reconstructions and
  * copy-out run on their own streams beside the rounds; device memory comes
