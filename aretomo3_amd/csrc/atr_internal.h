@@ -37,6 +37,7 @@ constexpr int FP_MAXP = 16;
 constexpr int FP_LDS_BUDGET = 150 * 1024;
 constexpr int FP_BZ_MAX = 17;                // cell rows per band at most
 constexpr int FP_STAGE_ROWS = 9;              // rows a thread loads before forming pairs
+constexpr int FP_RP_ISSUE = 8;                // register prefetch: pair slots before its loads
 // Band-prefetching variant: the last wave streams the next band's volume rows
 // into LDS with direct global->LDS loads while the other waves sample the
 // current band; the raw rows sit after the pair rows.  Needs VolX % 256 == 0

@@ -168,9 +168,9 @@ __device__ __forceinline__ void fp_raw_load(const ATR_GLOBAL_AS float* __restric
 
 // RP > 0: the register-prefetching variant (VolX 1024, 2048 without empty
 // voxels; RP = columns per thread, 1 or 2).  Each thread loads the
-// next band's rows of its columns into registers right after the current
-// band's pairs are in LDS, so the loads are in flight while the band is
-// sampled; the pairs are formed from the registers after the band.  Entry
+// next band's rows of its columns into registers while the current band is
+// sampled (after the wave's first FP_RP_ISSUE pair slots), and the pairs are
+// formed from the registers after the band.  Entry
 // VolX (x = VolX - 1 with dV = 0) is written by the owner of entry VolX - 1
 // from its second value, so no thread stages a column of its own beyond RP.
 template <typename S, bool SENT, int ROWB = 0, bool PF = false, int RP = 0>
@@ -326,7 +326,6 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
             if (rows_u == RP_ROWS) rp_write(rows_u, std::true_type{});
             else rp_write(rows_u, std::false_type{});
             __syncthreads();
-            if (bi + 1 < nb) rp_load_band(b + 1 < nb ? b + 1 : 0);
         } else if (!PF) {
             const int rows_u = __builtin_amdgcn_readfirstlane(rows);
             const int zoff = __builtin_amdgcn_readfirstlane(zb) * vol_x * (int)sizeof(float);
@@ -373,6 +372,11 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
         if (!PF && !RP) __syncthreads();
 #pragma unroll
         for (int i = 0; i < FP_MAXP; i++) {
+            // RP: the next band's loads are issued after the wave's first
+            // FP_RP_ISSUE pair slots: a masked window's interval load waits
+            // for every load issued before it, and the first slots' masked
+            // windows then do not wait for the prefetch
+            if (RP && i == FP_RP_ISSUE && bi + 1 < nb) rp_load_band(b + 1 < nb ? b + 1 : 0);
             const int jt = pair_tab[i];
             if (jt >= 0) {
                 const int t = fp_list[j0 + (jt >> 16)];
