@@ -389,8 +389,6 @@ void fp_band_candidates(int vol_x, int vol_z, std::vector<int>& rows, int pf)
     }
 }
 
-// (tilt-in-group, tile) of every pair index p = wave + 16 i of the forward
-// projector: j << 16 | tile.
 // Tilts per forward-projection workgroup for a list of n_list tilts: at most
 // the FP_WAVES * FP_MAXP pair slots of 64-ray tiles, and the list split into
 // groups of equal size (13 tilts with room for 8 -> 7 + 6, not 8 + 5).
@@ -412,9 +410,10 @@ int fp_group(int proj, int n_list, int n_waves)
 // apart, so bank conflicts grow with the tilt) and by 1.4 for masked
 // windows, plus a fixed 8 per non-empty descriptor.
 // tab gets one block of FP_WAVES * FP_MAXP entries per group of `group`
-// tilts: entry [w * FP_MAXP + i] = (local tilt << 16) | tile, or -1.  Only
-// waves 0 .. n_waves - 1 get pairs (the band-prefetching projector keeps its
-// last wave for the loads).
+// tilts: entry [w * FP_MAXP + i] = (local tilt << 24) | (tilt * ntile +
+// tile), the second part being the pair's row of band descriptors, or -1.
+// Only waves 0 .. n_waves - 1 get pairs (the band-prefetching projector
+// keeps its last wave for the loads).
 void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, int group,
                       std::vector<int>& tab, int n_waves)
 {
@@ -543,7 +542,8 @@ void fp_pair_schedule(const Geometry& g, const int* list, int n_list, int proj, 
         std::vector<int> slot(FP_WAVES, 0);
         for (int it = 0; it < n_items; it++) {
             const int w = owner[it];
-            tab[base + w * FP_MAXP + slot[w]++] = ((it / ntile) << 16) | (it % ntile);
+            const int t = list[j0 + it / ntile];
+            tab[base + w * FP_MAXP + slot[w]++] = ((it / ntile) << 24) | (t * ntile + it % ntile);
         }
     }
 }

@@ -197,7 +197,8 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int j0 = gx * group;
     const int ntile = (proj + FP_WAVE - 1) / FP_WAVE;
-    // this wave's (local tilt << 16 | tile) slots, -1 = unused
+    // this wave's slots: local tilt << 24 | pair index (tilt * ntile + tile,
+    // the row of the pair's band descriptors), -1 = unused
     const cptr<int> pair_tab = sched + (size_t)gx * (FP_WAVES * FP_MAXP) + wave * FP_MAXP;
     const int pitch = vol_x + 1;               // pairs per band row (guard + VolX)
     const int rowb = pitch * (int)sizeof(float2);
@@ -215,8 +216,9 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
         tf1[i] = 1.0f;
         const int jt = pair_tab[i];
         if (jt >= 0) {
-            const int t = fp_list[j0 + (jt >> 16)];
-            const int xd = min((jt & 0xffff) * FP_WAVE + lane, proj - 1);
+            const int t = fp_list[j0 + (jt >> 24)];
+            const int tile = (jt & 0xffffff) - t * ntile;
+            const int xd = min(tile * FP_WAVE + lane, proj - 1);
             tf1[i] = (float)(((double)xd - 0.5 * proj) / (double)cs[t].x + vol_x * 0.5 + 1.0);
         }
     }
@@ -379,9 +381,9 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
             if (RP && i == FP_RP_ISSUE && bi + 1 < nb) rp_load_band(b + 1 < nb ? b + 1 : 0);
             const int jt = pair_tab[i];
             if (jt >= 0) {
-                const int t = fp_list[j0 + (jt >> 16)];
-                const int tile = jt & 0xffff;
-                const int4 d = ldc(fp_desc + ((size_t)t * ntile + tile) * nb + b);
+                const int4 d = ldc(fp_desc + (size_t)(jt & 0xffffff) * nb + b);
+                // all four fields in one scalar load, before the count test
+                asm volatile("" ::"s"(d.x), "s"(d.w));
                 if (d.y > 0) {
                     const cptr<float4> G = geo + d.x;
                     if (!SENT && d.w != 0) {
@@ -393,6 +395,8 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
                                                    make_int2(0, 0), part, unused);
                         acc[i] += part;
                     } else {
+                        const int t = fp_list[j0 + (jt >> 24)];
+                        const int tile = (jt & 0xffffff) - t * ntile;
                         int xd = tile * FP_WAVE + lane;
                         // keep the interval load inside the band loop: hoisted,
                         // the intervals of all pairs would pin 2*FP_MAXP VGPRs
@@ -417,9 +421,9 @@ __device__ __forceinline__ void fp_block(cptr<FpArgs> A, float2* __restrict__ ba
     for (int i = 0; i < FP_MAXP; i++) {
         const int jt = pair_tab[i];
         if (jt >= 0) {
-            const int j = j0 + (jt >> 16);
+            const int j = j0 + (jt >> 24);
             const int t = fp_list[j];
-            const int xd = (jt & 0xffff) * FP_WAVE + lane;
+            const int xd = ((jt & 0xffffff) - t * ntile) * FP_WAVE + lane;
             if (xd < proj) {
                 const int count = SENT ? cnt[i] : fp_range[(size_t)t * proj + xd].y;
                 const float fwd = count <= 0 ? kSentinel : acc[i] / (float)count;
