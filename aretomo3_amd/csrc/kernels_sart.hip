@@ -323,8 +323,10 @@ __device__ __forceinline__ void bp_tile_body(cptr<BpArgs> A, float2 (&pairs)[CH]
             const TiltRow tr = ldc(list + c0 + j);
             // {w0, class, u - w0 at the tile's first voxel}
             const int4 tw = ldc(tile_tab + (size_t)tr.tilt * tilt_step);
-            if (tw.y == 0) continue;
             const float2 sc = ldc(cs + tr.tilt);
+            // both loads issued together, before the class test
+            asm volatile("" ::"s"(tw.z), "s"(sc.x), "s"(sc.y));
+            if (tw.y == 0) continue;
             // scale is 1 or 1/2: both products are exact
             const float cc = scale * sc.x, ss = scale * sc.y;
             const float u0 = fmaf(xf, cc, __int_as_float(tw.z));
